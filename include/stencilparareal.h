@@ -1,0 +1,202 @@
+/*
+ * stencilparareal.h -- C ABI of the MI355X (gfx950) StencilParareal hot path.
+ *
+ * One shared library, libsp_hip.so, built from stencilparareal_amd/csrc/.
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (SP_OK == 0) and never exits the process; sp_last_error() gives a
+ * thread-local message for the last failure.  Device pointers are HIP device
+ * memory; all work is enqueued on the context's stream (asynchronous unless the
+ * function says otherwise).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   Convection ctor ................ src/Convection/Convection.h:25
+ *   Convection::DoRK4 / DoEuler .... src/Convection/Convection.h:44-45, :63-64
+ *   DoRK4Timestep / DoEulerTimestep  src/Convection/Convection.h:69-75
+ *   ApplyImpl (periodic halo fill) . src/Communication/Periodicity.h:103
+ *   UpdateStage (Parareal update) .. src/Parareal/Parareal.h:19-35
+ *   Parareal::DoParallel / DoSerial  src/Parareal/Parareal.h:110-171, :250-253
+ *   Parareal::DoTimedParallel ...... src/Parareal/Parareal.h:173-248
+ *   computeErrorReference .......... src/Parareal/main.cpp:98-139
+ *   fillQ (initial condition) ...... src/Convection/ConvectionSolution.h:23-52
+ */
+#ifndef STENCILPARAREAL_H_
+#define STENCILPARAREAL_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define SP_OK 0
+#define SP_ERR_INVALID 1     /* bad argument: null pointer, size, layout, steps < 0 */
+#define SP_ERR_HIP 2         /* a HIP runtime call failed */
+#define SP_ERR_UNSUPPORTED 3 /* valid request this build does not implement */
+#define SP_ERR_DIVERGED 4    /* non-finite or growing Parareal increment */
+#define SP_ERR_NOMEM 5       /* device allocation failed */
+
+/* ---- element types ----------------------------------------------------------- */
+#define SP_F64 0
+#define SP_F32 1
+#define SP_BF16 2 /* storage only; arithmetic in f32 */
+
+/* ---- coarse propagator kinds -------------------------------------------------- */
+#define SP_COARSE_EULER 0 /* DoEuler: Advection2 + Laplace2 (upwind per sp_prop_set_upwind) */
+#define SP_COARSE_RK4 1   /* DoRK4 with the coarse step */
+
+/* ---- kernel paths for the fine propagator ------------------------------------- */
+#define SP_PATH_FUSED 0  /* one kernel per RK4 step, 4 stages fused (default) */
+#define SP_PATH_STAGED 1 /* reference structure: 4 RHS + 3 Euler + 1 RK sweep per step */
+
+/*
+ * Field layout.  Element (i,j,k) of the interior, 0 <= i < ni etc., lives at
+ *   base[origin + i + j*sj + k*sk]
+ * (i has unit stride).  halo is the number of ghost cells present on every side
+ * in the storage; the propagators never read or write halos (periodicity is
+ * index arithmetic), sp_periodic_fill writes them.
+ *   compact layout:   halo 0, sj = ni, sk = ni*nj, origin 0
+ *   STELLA CUDA ConvectionField (KJI order, i padded to 32 doubles, 2-cell halo;
+ *   src/Convection/Convection.h:17; STELLA DataFieldStorageFormat.h:146-174):
+ *                     halo 2, pi = pad32(ni+4), sj = pi, sk = pi*(nj+4),
+ *                     origin = 2 + 2*sj + 2*sk (base = pStorageBase)
+ */
+typedef struct sp_layout {
+    int ni, nj, nk;
+    int halo;
+    long long sj, sk;
+    long long origin;
+    int dtype; /* SP_F64 / SP_F32 / SP_BF16 */
+} sp_layout;
+
+typedef struct sp_ctx sp_ctx;
+typedef struct sp_prop sp_prop;
+typedef struct sp_parareal sp_parareal;
+
+/* ---- library and context ------------------------------------------------------- */
+const char* sp_version(void);
+const char* sp_last_error(void);
+/* Opens device `device`, creates a stream of default priority. */
+int sp_ctx_create(int device, sp_ctx** out);
+int sp_ctx_destroy(sp_ctx* ctx);
+/* Enqueue on an external hipStream_t; NULL selects HIP's null stream.  The context's
+ * own stream is what sp_ctx_get_stream returns right after sp_ctx_create. */
+int sp_ctx_set_stream(sp_ctx* ctx, void* hip_stream);
+int sp_ctx_get_stream(sp_ctx* ctx, void** hip_stream);
+/* Blocks until all work enqueued on the context's stream has finished. */
+int sp_ctx_sync(sp_ctx* ctx);
+int sp_malloc(sp_ctx* ctx, long long bytes, void** out);
+int sp_free(sp_ctx* ctx, void* ptr);
+/* Synchronous copies between host and device memory. */
+int sp_memcpy_h2d(sp_ctx* ctx, void* dst, const void* src, long long bytes);
+int sp_memcpy_d2h(sp_ctx* ctx, void* dst, const void* src, long long bytes);
+
+int sp_layout_compact(int ni, int nj, int nk, int dtype, sp_layout* out);
+int sp_layout_stella_cuda(int ni, int nj, int nk, sp_layout* out);
+/* Number of elements the layout's storage spans (allocation size). */
+long long sp_layout_storage_elems(const sp_layout* l);
+
+/* ---- propagator: Convection (src/Convection/Convection.h:22-108) -------------- */
+/* Convection(isize, jsize, ksize, dx, nu0, nufreq, cx, cy, cz).  nu(t) =
+ * nu0*(1 + sin(nufreq*t)/2) is evaluated once per step, at the step start. */
+int sp_prop_create(sp_ctx* ctx, int isize, int jsize, int ksize, double dx, double nu0,
+                   double nufreq, double cx, double cy, double cz, sp_prop** out);
+int sp_prop_destroy(sp_prop* p);
+/* Advection2 branch of DoEuler: 1 = first-order upwind (the reference build
+ * default, CMakeLists.txt:101-105), 0 = second-order central. */
+int sp_prop_set_upwind(sp_prop* p, int upwind);
+/* SP_PATH_FUSED (default) or SP_PATH_STAGED for DoRK4. */
+int sp_prop_set_path(sp_prop* p, int path);
+/* Fused-kernel tile: tx, ty in {16, 24, 32}, nthreads in {256, 512, 1024}, kchunk
+ * in 1..nk (0 for the autotuned choice). */
+int sp_prop_set_tile(sp_prop* p, int tx, int ty, int nthreads, int kchunk);
+
+/* DoRK4: `steps` RK4 steps of size dt from tStart; step i runs at tStart + i*dt.
+ * in == out is allowed.  steps == 0 leaves out untouched.  in and out may have
+ * different layouts; their dtype must be SP_F64 (the fine propagator) or match
+ * the coarse precision, see sp_do_rk4_ex. */
+int sp_do_rk4(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
+              double tStart, double dt, int steps);
+/* DoEuler: forward Euler with the RHS2 stencil (SP_F64 only). */
+int sp_do_euler(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
+                double tStart, double dt, int steps);
+int sp_do_rk4_timestep(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout,
+                       void* out, double t, double dt);
+int sp_do_euler_timestep(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout,
+                         void* out, double t, double dt);
+/* RK4 with arbitrary storage types: in/out dtypes from the layouts
+ * (f64/f32/bf16); arithmetic in f64 when both are f64, in f32 otherwise. */
+int sp_do_rk4_ex(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
+                 double tStart, double dt, int steps);
+
+/* ---- halo fill, update, norms -------------------------------------------------- */
+/* ApplyImpl: fills the layout's halo periodically, I faces (interior j,k), then J
+ * faces (all i, interior k), then K faces (all i, j).  SP_F64, halo >= 1. */
+int sp_periodic_fill(sp_ctx* ctx, const sp_layout* l, void* base);
+/* UpdateStage: q = cnew + fine - cold on the interior (all SP_F64, same layout). */
+int sp_parareal_update(sp_ctx* ctx, const sp_layout* l, void* q, const void* fine,
+                       const void* cnew, const void* cold);
+/* out[0] = max|a-b|, out[1] = max|a|, out[2] = sum (a-b)^2, out[3] = sum b^2 over
+ * the interior (SP_F64).  Synchronous.  computeErrorReference = out[0]/out[1];
+ * relative L2 = sqrt(out[2]/out[3]). */
+int sp_norms(sp_ctx* ctx, const sp_layout* la, const void* a, const sp_layout* lb, const void* b,
+             double out[4]);
+/* fillQ with exactQ at time t on the unit cube (cell-centred points). */
+int sp_fill_exact(sp_ctx* ctx, const sp_layout* l, void* base, double nu0, double nufreq,
+                  double cx, double cy, double cz, double t);
+/* Sum of ng Gaussians amp*exp(-d^2/(2 sigma^2)), d the periodic minimum-image
+ * distance to the centre; host arrays of length ng. */
+int sp_fill_gaussians(sp_ctx* ctx, const sp_layout* l, void* base, int ng, const double* xc,
+                      const double* yc, const double* zc, const double* sigma, const double* amp);
+
+/* ---- Parareal (src/Parareal/Parareal.h) --------------------------------------- */
+/* The S time slices the reference gives one MPI rank each are the batch index of
+ * one GPU: the fine propagation of all slices runs as one launch per step, the
+ * coarse correction chain runs slice after slice, the update is fused with an
+ * increment norm.  Step totals are rounded up to multiples of S as in
+ * RuntimeConfiguration.cpp:259-277; dt = T / total steps. */
+typedef struct sp_parareal_cfg {
+    int slices;          /* S (the reference's MPI world size) */
+    int kmax;            /* maximum iterations */
+    double end_time;     /* T */
+    int steps_fine;      /* total fine steps over [0, T] */
+    int steps_coarse;    /* total coarse steps over [0, T] */
+    int coarse_kind;     /* SP_COARSE_EULER (reference) or SP_COARSE_RK4 */
+    int coarse_dtype;    /* SP_F64, SP_F32 or SP_BF16 (RK4 only for the latter two) */
+    int coarse_substeps; /* >= 1: each coarse step is this many sub-steps */
+    double tol;          /* stop once the relative L2 increment <= tol; 0: run kmax */
+    int async_schedule;  /* 1: fine-step times of the reference's async mode */
+    int use_graphs;      /* 1: capture each iteration's fine sweep as a hipGraph */
+} sp_parareal_cfg;
+
+#define SP_MAX_ITERS 64
+typedef struct sp_parareal_report {
+    int iterations;                    /* iterations actually run */
+    int steps_fine_per_slice, steps_coarse_per_slice;
+    double dt_fine, dt_coarse;         /* effective steps after rounding */
+    double increment[SP_MAX_ITERS];    /* relative L2 change of the slice states */
+    double t_init;                     /* seconds: initial coarse sweep */
+    double t_fine[SP_MAX_ITERS];       /* seconds per iteration: batched fine sweep */
+    double t_coarse[SP_MAX_ITERS];     /* coarse chain incl. updates */
+    double t_total;
+} sp_parareal_report;
+
+int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out);
+int sp_parareal_destroy(sp_parareal* pr);
+/* Initialisation of DoParallel (Parareal.h:113-114): slice states U_n = G^n(u0),
+ * coarse values G(U_n).  u0: SP_F64 field on the device. */
+int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0);
+/* One iteration (Parareal.h:116-165).  *increment (nullable) receives the
+ * relative L2 change of all slice states.  Synchronous. */
+int sp_parareal_iterate(sp_parareal* pr, double* increment);
+/* Copies the current last-slice solution (the last rank's q) into q. */
+int sp_parareal_solution(sp_parareal* pr, const sp_layout* lq, void* q);
+/* DoParallel: begin + up to kmax iterations (stopping at tol) + solution. */
+int sp_parareal_run(sp_parareal* pr, const sp_layout* l0, const void* u0, const sp_layout* lq,
+                    void* q, sp_parareal_report* rep);
+/* DoSerial: fine propagation over all slices, DoRK4(u0 -> q, 0, dtF, tsF*S). */
+int sp_parareal_serial(sp_parareal* pr, const sp_layout* l0, const void* u0, const sp_layout* lq,
+                       void* q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STENCILPARAREAL_H_ */

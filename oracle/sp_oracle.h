@@ -1,0 +1,91 @@
+/*
+ * sp_oracle.h -- CPU restatement of the StencilParareal numerics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product library (stencilparareal_amd/libsp_hip.so) never links
+ * or calls it.
+ *
+ * Fields follow the reference OpenMP backend storage: a 2-cell halo on every
+ * side and JIK storage order, i.e. k has unit stride, then i, then j
+ * (src/Convection/Convection.h:13-20; SharedInfrastructure.h:76).
+ * Exchange arrays ("compact") are interior-only, i fastest: a[(k*nj + j)*ni + i].
+ */
+#ifndef SP_ORACLE_H_
+#define SP_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_field orc_field;
+typedef struct orc_conv orc_conv;
+
+/* ---- fields -------------------------------------------------------------- */
+orc_field* orc_field_new(int ni, int nj, int nk);
+void orc_field_free(orc_field* f);
+void orc_field_set(orc_field* f, const double* compact);
+void orc_field_get(const orc_field* f, double* compact);
+/* whole storage (halo included), JIK order, (ni+4)(nj+4)(nk+4) doubles */
+void orc_field_get_storage(const orc_field* f, double* storage);
+void orc_field_set_storage(orc_field* f, const double* storage);
+void orc_field_copy(orc_field* dst, const orc_field* src);
+
+/* Periodicity ApplyImpl restated (src/Communication/Periodicity.cpp:100-135):
+ * generic strided storage, I faces then J faces then K faces. */
+void orc_apply_periodic(double* base, int ni, int nj, int nk,
+                        int him, int hip, int hjm, int hjp, int hkm, int hkp,
+                        int si, int sj, int sk);
+void orc_field_periodic(orc_field* f);
+
+/* ---- propagator (src/Convection/Convection.{h,cpp}) ---------------------- */
+/* upwind != 0 selects the UPWINDADVECTION Advection2 branch (reference default,
+ * CMakeLists.txt:101-105); upwind == 0 the central branch. */
+orc_conv* orc_conv_new(int ni, int nj, int nk, double dx, double nu0, double nufreq,
+                       double cx, double cy, double cz, int upwind);
+void orc_conv_free(orc_conv* c);
+void orc_do_rk4(orc_conv* c, orc_field* in, orc_field* out, double tStart, double dt, int steps);
+void orc_do_euler(orc_conv* c, orc_field* in, orc_field* out, double tStart, double dt, int steps);
+void orc_do_rk4_timestep(orc_conv* c, orc_field* in, orc_field* out, double t, double dt);
+void orc_do_euler_timestep(orc_conv* c, orc_field* in, orc_field* out, double t, double dt);
+/* single right-hand-side sweeps (k = RHS(q)); q's halo is filled first */
+void orc_rhs4(orc_conv* c, orc_field* q, orc_field* k, double nu);
+void orc_rhs2(orc_conv* c, orc_field* q, orc_field* k, double nu);
+double orc_computenu(const orc_conv* c, double t);
+
+/* ---- Parareal (src/Parareal/Parareal.h) --------------------------------- */
+/* g_kind: 0 = Euler (reference DoEuler; Advection2 per the conv's upwind flag),
+ *         1 = RK4 with the coarse step.
+ * Emulates DoParallel on S ranks in rank order.  u0: compact initial field.
+ * q_iter (nullable): (kmax+1) compact fields; entry k is the last rank's q
+ * after k iterations (entry 0: the coarse prediction G^{S*tsG}(u0)).
+ * q_out: compact final field of the last rank. */
+void orc_parareal_parallel(orc_conv* c, int S, int kmax, double T,
+                           double dtF, int tsF, double dtG, int tsG, int g_kind,
+                           const double* u0, double* q_iter, double* q_out);
+/* DoSerial: DoRK4(u0 -> q, 0, dtF, tsF*S) (Parareal.h:250-253) */
+void orc_parareal_serial(orc_conv* c, double dtF, int steps_total,
+                         const double* u0, double* q_out);
+
+/* ---- norms and initial conditions --------------------------------------- */
+/* computeErrorReference (src/Parareal/main.cpp:98-139): max|f-r| / max|f| */
+double orc_error_reference(const double* f, const double* r, long n);
+/* relative L2: ||f-r||_2 / ||r||_2 */
+double orc_rel_l2(const double* f, const double* r, long n);
+/* exactQ / fillQ (src/Convection/ConvectionSolution.h:8-52), unit cube */
+double orc_exactq(double nu0, double nufreq, double cx, double cy, double cz,
+                  double x, double y, double z, double t);
+void orc_fill_exact(double* compact, int ni, int nj, int nk, double nu0, double nufreq,
+                    double cx, double cy, double cz, double t);
+/* sum of Gaussians, periodic minimum-image distance, cell-centred points */
+void orc_fill_gaussians(double* compact, int ni, int nj, int nk, int ng,
+                        const double* xc, const double* yc, const double* zc,
+                        const double* sigma, const double* amp);
+
+double orc_wtime(void);
+int orc_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,82 @@
+"""`Convection`: the advection-diffusion propagator, mirroring the reference class
+(src/Convection/Convection.h:22-108) on top of libsp_hip.so."""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib as L
+from .field import Context, ConvectionField
+
+
+class Convection:
+    """Convection(isize, jsize, ksize, dx, nu0, nufreq, cx, cy, cz) (Convection.h:25).
+
+    upwind: Advection2 branch of DoEuler (True = the reference build's default,
+    CMakeLists.txt:101-105).  path: "fused" (one kernel per RK4 step) or "staged"
+    (the reference's 4 RHS + 3 Euler + 1 RK sweeps, bit-for-bit its operation order).
+    """
+
+    def __init__(self, isize, jsize, ksize, dx, nu0, nufreq, cx, cy, cz, *, upwind=True,
+                 path="fused", ctx: Context | None = None, device: int = 0):
+        self.ctx = ctx or Context.default(device)
+        self.sizes = (isize, jsize, ksize)
+        self.dx, self.nu0, self.nufreq = dx, nu0, nufreq
+        self.cx, self.cy, self.cz = cx, cy, cz
+        h = C.c_void_p()
+        L.call("sp_prop_create", self.ctx.h, isize, jsize, ksize, dx, nu0, nufreq, cx, cy, cz, C.byref(h))
+        self.h = h
+        self.set_upwind(upwind)
+        self.set_path(path)
+
+    def set_upwind(self, upwind: bool):
+        L.call("sp_prop_set_upwind", self.h, 1 if upwind else 0)
+
+    def set_path(self, path: str):
+        L.call("sp_prop_set_path", self.h, {"fused": L.SP_PATH_FUSED, "staged": L.SP_PATH_STAGED}[path])
+
+    def set_tile(self, tx=0, ty=0, nthreads=0, kchunk=0):
+        L.call("sp_prop_set_tile", self.h, tx, ty, nthreads, kchunk)
+
+    def computenu(self, t: float) -> float:
+        import math
+        return self.nu0 * (1. + .5 * math.sin(self.nufreq * t))
+
+    def field(self, name="q", kind="compact", dtype="f64") -> ConvectionField:
+        ni, nj, nk = self.sizes
+        return ConvectionField(name, ni, nj, nk, kind=kind, dtype=dtype,
+                               device=self.ctx.device)
+
+    # -- the reference interface ---------------------------------------------------
+    def DoRK4(self, inputField: ConvectionField, outputField: ConvectionField, tStart: float,
+              dt: float, timesteps: int):
+        """Convection.h:44-45 / Convection.cpp:292-303."""
+        self.ctx.sync_stream()
+        fn = "sp_do_rk4" if inputField.dtype == outputField.dtype == L.SP_F64 else "sp_do_rk4_ex"
+        L.call(fn, self.h, inputField.lp(), inputField.vp(), outputField.lp(), outputField.vp(),
+               tStart, dt, timesteps)
+
+    def DoEuler(self, inputField: ConvectionField, outputField: ConvectionField, tStart: float,
+                dt: float, timesteps: int):
+        """Convection.h:63-64 / Convection.cpp:305-316."""
+        self.ctx.sync_stream()
+        L.call("sp_do_euler", self.h, inputField.lp(), inputField.vp(), outputField.lp(),
+               outputField.vp(), tStart, dt, timesteps)
+
+    def DoRK4Timestep(self, inputField, outputField, t, dt):
+        """Convection.h:69-70 / Convection.cpp:318-370."""
+        self.ctx.sync_stream()
+        L.call("sp_do_rk4_timestep", self.h, inputField.lp(), inputField.vp(), outputField.lp(),
+               outputField.vp(), t, dt)
+
+    def DoEulerTimestep(self, inputField, outputField, t, dt):
+        """Convection.h:74-75 / Convection.cpp:372-392."""
+        self.ctx.sync_stream()
+        L.call("sp_do_euler_timestep", self.h, inputField.lp(), inputField.vp(), outputField.lp(),
+               outputField.vp(), t, dt)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                L.load().sp_prop_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,1012 @@
+// sp_api.hip -- the C ABI (include/stencilparareal.h): contexts, the Convection
+// propagator, the periodic fill, the Parareal update and the batched single-GPU
+// Parareal driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sp_internal.hpp"
+
+namespace sp {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& m) { g_err = m; }
+int fail(int code, const std::string& m)
+{
+    g_err = m;
+    return code;
+}
+
+int check_layout(const sp_layout* l, const char* what)
+{
+    SP_CHECK(l, SP_ERR_INVALID, std::string(what) + ": null layout");
+    SP_CHECK(l->ni >= 2 && l->nj >= 2 && l->nk >= 2, SP_ERR_INVALID, std::string(what) + ": sizes must be >= 2");
+    SP_CHECK(l->halo >= 0, SP_ERR_INVALID, std::string(what) + ": negative halo");
+    SP_CHECK(l->sj >= l->ni + 2LL * l->halo && l->sk >= l->sj * (l->nj + 2LL * l->halo), SP_ERR_INVALID,
+             std::string(what) + ": strides smaller than the padded sizes");
+    SP_CHECK(l->origin >= l->halo * (1 + l->sj + l->sk), SP_ERR_INVALID, std::string(what) + ": origin inside halo");
+    SP_CHECK(l->dtype == SP_F64 || l->dtype == SP_F32 || l->dtype == SP_BF16, SP_ERR_INVALID,
+             std::string(what) + ": unknown dtype");
+    SP_CHECK(l->sk * (l->nk + 2LL * l->halo) < (1LL << 31), SP_ERR_UNSUPPORTED,
+             std::string(what) + ": field exceeds 2^31 elements");
+    return SP_OK;
+}
+
+}  // namespace sp
+
+using namespace sp;
+
+struct sp_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    double* red = nullptr;       // device partials
+    double* red_host = nullptr;  // pinned host copy
+    int red_cap = 0;             // doubles
+};
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap) return SP_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess) return fail(SP_ERR_NOMEM, "hipMalloc of scratch failed");
+        cap = bytes;
+        return SP_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct sp_prop {
+    sp_ctx* ctx = nullptr;
+    int ni = 0, nj = 0, nk = 0;
+    double dx = 0, nu0 = 0, nufreq = 0, cx = 0, cy = 0, cz = 0;
+    int upwind = 1;
+    int path = SP_PATH_FUSED;
+    FusedTile tile{0, 0, 0, -1};  // tx == 0: automatic
+    Buf scratch[2];                // ping-pong for the fused steppers
+    Buf qI, k1, k2, k3, k4;         // staged path temporaries (f64, compact)
+    Buf nu_tab;                     // per-step nu table
+    std::vector<double> nu_host;
+};
+
+namespace {
+
+double computenu(const sp_prop* p, double t) { return p->nu0 * (1. + .5 * std::sin(p->nufreq * t)); }
+
+sp_layout compact(int ni, int nj, int nk, int dt)
+{
+    sp_layout l;
+    l.ni = ni;
+    l.nj = nj;
+    l.nk = nk;
+    l.halo = 0;
+    l.sj = ni;
+    l.sk = (long long)ni * nj;
+    l.origin = 0;
+    l.dtype = dt;
+    return l;
+}
+
+View cview(void* p, int ni, int nj) { return View{p, ni, (long long)ni * nj, 0}; }
+
+bool same_origin(const sp_layout* la, const void* a, const sp_layout* lb, const void* b)
+{
+    return (const char*)a + la->origin * dtype_size(la->dtype) == (const char*)b + lb->origin * dtype_size(lb->dtype);
+}
+
+int check_prop_layout(const sp_prop* p, const sp_layout* l, const void* ptr, const char* what)
+{
+    SP_TRY(check_layout(l, what));
+    SP_CHECK(ptr, SP_ERR_INVALID, std::string(what) + ": null pointer");
+    SP_CHECK(l->ni == p->ni && l->nj == p->nj && l->nk == p->nk, SP_ERR_INVALID,
+             std::string(what) + ": size differs from the propagator's");
+    return SP_OK;
+}
+
+FusedTile pick_tile(const sp_prop* p, int nslices, int f64)
+{
+    FusedTile t = default_tile(p->ni, p->nj, p->nk, nslices, f64);
+    if (p->tile.tx > 0) {
+        t.tx = p->tile.tx;
+        t.ty = p->tile.ty;
+        t.nt = p->tile.nt;
+    }
+    if (p->tile.kchunk >= 0) t.kchunk = p->tile.kchunk;
+    return t;
+}
+
+FusedStep fused_step(const sp_prop* p, int nslices, double dt, const double* nu_dev, double nu)
+{
+    FusedStep st;
+    st.ni = p->ni;
+    st.nj = p->nj;
+    st.nk = p->nk;
+    st.nslices = nslices;
+    st.dt = dt;
+    st.cx = p->cx;
+    st.cy = p->cy;
+    st.cz = p->cz;
+    st.dx = p->dx;
+    st.nu_dev = nu_dev;
+    st.nu = nu;
+    return st;
+}
+
+// RK4 steps with the fused kernel on one field; in/out views and dtypes given.
+int rk4_fused_run(sp_prop* p, int idt, View vin, int odt, View vout, bool aliased, double t0, double dt, int steps)
+{
+    hipStream_t s = p->ctx->stream;
+    const size_t n = (size_t)p->ni * p->nj * p->nk;
+    const int sdt = odt;  // intermediate storage precision = output precision
+    const size_t bytes = n * dtype_size(sdt);
+    SP_TRY(p->scratch[0].ensure(bytes));
+    if (aliased) SP_TRY(p->scratch[1].ensure(bytes));
+    const FusedTile tile = pick_tile(p, 1, idt == SP_F64 && odt == SP_F64);
+    const View A = cview(p->scratch[0].p, p->ni, p->nj), B = cview(p->scratch[1].p, p->ni, p->nj);
+    View src = vin;
+    int sdtype = idt;
+    for (int i = 0; i < steps; ++i) {
+        View dst;
+        int ddt = sdt;
+        const int left = steps - 1 - i;  // steps remaining after this one
+        if (!aliased) {
+            dst = (left % 2 == 0) ? vout : A;
+        } else {
+            if (left == 0 && steps > 1) dst = vout;
+            else dst = (i % 2 == 0) ? A : B;
+        }
+        const double t = t0 + i * dt;  // Convection.cpp:302: tStart + (++i)*dt
+        const double nu = computenu(p, t);
+        SP_TRY(launch_rk4_fused(s, fused_step(p, 1, dt, nullptr, nu), tile, sdtype, src, ddt, dst));
+        src = dst;
+        sdtype = ddt;
+    }
+    if (aliased && steps == 1) SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, sdt, A, odt, vout));
+    return SP_OK;
+}
+
+// One reference-structured RK4 step (Convection.cpp:318-370), f64, in == out allowed.
+int rk4_staged_step(sp_prop* p, View vin, View vout, double t, double dt)
+{
+    hipStream_t s = p->ctx->stream;
+    const size_t bytes = (size_t)p->ni * p->nj * p->nk * 8;
+    SP_TRY(p->qI.ensure(bytes));
+    SP_TRY(p->k1.ensure(bytes));
+    SP_TRY(p->k2.ensure(bytes));
+    SP_TRY(p->k3.ensure(bytes));
+    SP_TRY(p->k4.ensure(bytes));
+    const StagedParams P{p->ni, p->nj, p->nk, p->cx, p->cy, p->cz, p->dx, p->dx * p->dx};
+    const View qI = cview(p->qI.p, p->ni, p->nj), k1 = cview(p->k1.p, p->ni, p->nj), k2 = cview(p->k2.p, p->ni, p->nj),
+               k3 = cview(p->k3.p, p->ni, p->nj), k4 = cview(p->k4.p, p->ni, p->nj);
+    const double dthalf = dt * .5;
+    const double nu = computenu(p, t);
+    SP_TRY(launch_rhs4(s, P, vin, k1, nu));
+    SP_TRY(launch_euler_stage(s, P, vin, k1, qI, dthalf));
+    SP_TRY(launch_rhs4(s, P, qI, k2, nu));
+    SP_TRY(launch_euler_stage(s, P, vin, k2, qI, dthalf));
+    SP_TRY(launch_rhs4(s, P, qI, k3, nu));
+    SP_TRY(launch_euler_stage(s, P, vin, k3, qI, dt));
+    SP_TRY(launch_rhs4(s, P, qI, k4, nu));
+    SP_TRY(launch_rk_stage(s, P, vin, k1, k2, k3, k4, vout, dt));
+    return SP_OK;
+}
+
+// Forward Euler steps (DoEuler), f64, with ping-pong so the fused sweep never runs in place.
+int euler_run(sp_prop* p, View vin, View vout, bool aliased, double t0, double dt, int steps, int nslices,
+              View scrA, View scrB)
+{
+    hipStream_t s = p->ctx->stream;
+    const StagedParams P{p->ni, p->nj, p->nk, p->cx, p->cy, p->cz, p->dx, p->dx * p->dx};
+    View src = vin;
+    for (int i = 0; i < steps; ++i) {
+        const int left = steps - 1 - i;
+        View dst;
+        if (!aliased) dst = (left % 2 == 0) ? vout : scrA;
+        else if (left == 0 && steps > 1) dst = vout;
+        else dst = (i % 2 == 0) ? scrA : scrB;
+        const double nu = computenu(p, t0 + i * dt);
+        SP_TRY(launch_euler_fused(s, P, src, dst, nu, dt, p->upwind, nslices));
+        src = dst;
+    }
+    if (aliased && steps == 1) SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, nslices, SP_F64, scrA, SP_F64, vout));
+    return SP_OK;
+}
+
+}  // namespace
+
+// ======================================================================================
+extern "C" {
+
+const char* sp_version(void) { return "stencilparareal-gfx950 0.1"; }
+const char* sp_last_error(void) { return g_err.c_str(); }
+
+int sp_ctx_create(int device, sp_ctx** out)
+{
+    SP_CHECK(out, SP_ERR_INVALID, "sp_ctx_create: null out");
+    int n = 0;
+    SP_HIP(hipGetDeviceCount(&n));
+    SP_CHECK(device >= 0 && device < n, SP_ERR_INVALID, "sp_ctx_create: no such device");
+    SP_HIP(hipSetDevice(device));
+    sp_ctx* c = new sp_ctx;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SP_ERR_HIP, "hipStreamCreate failed");
+    }
+    c->stream = c->own;
+    *out = c;
+    return SP_OK;
+}
+
+int sp_ctx_destroy(sp_ctx* c)
+{
+    if (!c) return SP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->red) (void)hipFree(c->red);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+    return SP_OK;
+}
+
+int sp_ctx_set_stream(sp_ctx* c, void* s)
+{
+    SP_CHECK(c, SP_ERR_INVALID, "sp_ctx_set_stream: null context");
+    c->stream = (hipStream_t)s;  // NULL: HIP's null stream
+    return SP_OK;
+}
+
+int sp_ctx_get_stream(sp_ctx* c, void** s)
+{
+    SP_CHECK(c && s, SP_ERR_INVALID, "sp_ctx_get_stream: null argument");
+    *s = (void*)c->stream;
+    return SP_OK;
+}
+
+int sp_ctx_sync(sp_ctx* c)
+{
+    SP_CHECK(c, SP_ERR_INVALID, "sp_ctx_sync: null context");
+    SP_HIP(hipStreamSynchronize(c->stream));
+    return SP_OK;
+}
+
+int sp_malloc(sp_ctx* c, long long bytes, void** out)
+{
+    SP_CHECK(c && out && bytes > 0, SP_ERR_INVALID, "sp_malloc: bad argument");
+    SP_HIP(hipSetDevice(c->device));
+    if (hipMalloc(out, (size_t)bytes) != hipSuccess) return fail(SP_ERR_NOMEM, "sp_malloc: out of device memory");
+    return SP_OK;
+}
+
+int sp_free(sp_ctx* c, void* p)
+{
+    SP_CHECK(c, SP_ERR_INVALID, "sp_free: null context");
+    if (p) SP_HIP(hipFree(p));
+    return SP_OK;
+}
+
+int sp_memcpy_h2d(sp_ctx* c, void* dst, const void* src, long long bytes)
+{
+    SP_CHECK(c && dst && src && bytes >= 0, SP_ERR_INVALID, "sp_memcpy_h2d: bad argument");
+    SP_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    SP_HIP(hipStreamSynchronize(c->stream));
+    return SP_OK;
+}
+
+int sp_memcpy_d2h(sp_ctx* c, void* dst, const void* src, long long bytes)
+{
+    SP_CHECK(c && dst && src && bytes >= 0, SP_ERR_INVALID, "sp_memcpy_d2h: bad argument");
+    SP_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    SP_HIP(hipStreamSynchronize(c->stream));
+    return SP_OK;
+}
+
+int sp_layout_compact(int ni, int nj, int nk, int dtype, sp_layout* out)
+{
+    SP_CHECK(out, SP_ERR_INVALID, "sp_layout_compact: null out");
+    *out = compact(ni, nj, nk, dtype);
+    return check_layout(out, "sp_layout_compact");
+}
+
+int sp_layout_stella_cuda(int ni, int nj, int nk, sp_layout* out)
+{
+    SP_CHECK(out, SP_ERR_INVALID, "sp_layout_stella_cuda: null out");
+    const int h = 2;
+    const long long pi = ((ni + 2 * h + 31) / 32) * 32LL;
+    out->ni = ni;
+    out->nj = nj;
+    out->nk = nk;
+    out->halo = h;
+    out->sj = pi;
+    out->sk = pi * (nj + 2 * h);
+    out->origin = h + h * out->sj + h * out->sk;
+    out->dtype = SP_F64;
+    return check_layout(out, "sp_layout_stella_cuda");
+}
+
+long long sp_layout_storage_elems(const sp_layout* l)
+{
+    if (!l) return -1;
+    return l->origin + (l->ni + l->halo) + (long long)(l->nj - 1 + l->halo) * l->sj +
+           (long long)(l->nk - 1 + l->halo) * l->sk;
+}
+
+int sp_prop_create(sp_ctx* c, int isize, int jsize, int ksize, double dx, double nu0, double nufreq, double cx,
+                   double cy, double cz, sp_prop** out)
+{
+    SP_CHECK(c && out, SP_ERR_INVALID, "sp_prop_create: null argument");
+    SP_CHECK(isize >= 2 && jsize >= 2 && ksize >= 2, SP_ERR_INVALID, "sp_prop_create: sizes must be >= 2");
+    SP_CHECK(dx > 0 && std::isfinite(dx), SP_ERR_INVALID, "sp_prop_create: dx must be positive");
+    sp_prop* p = new sp_prop;
+    p->ctx = c;
+    p->ni = isize;
+    p->nj = jsize;
+    p->nk = ksize;
+    p->dx = dx;
+    p->nu0 = nu0;
+    p->nufreq = nufreq;
+    p->cx = cx;
+    p->cy = cy;
+    p->cz = cz;
+    *out = p;
+    return SP_OK;
+}
+
+int sp_prop_destroy(sp_prop* p)
+{
+    if (!p) return SP_OK;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    for (Buf* b : {&p->scratch[0], &p->scratch[1], &p->qI, &p->k1, &p->k2, &p->k3, &p->k4, &p->nu_tab}) b->release();
+    delete p;
+    return SP_OK;
+}
+
+int sp_prop_set_upwind(sp_prop* p, int upwind)
+{
+    SP_CHECK(p, SP_ERR_INVALID, "sp_prop_set_upwind: null propagator");
+    p->upwind = upwind ? 1 : 0;
+    return SP_OK;
+}
+
+int sp_prop_set_path(sp_prop* p, int path)
+{
+    SP_CHECK(p, SP_ERR_INVALID, "sp_prop_set_path: null propagator");
+    SP_CHECK(path == SP_PATH_FUSED || path == SP_PATH_STAGED, SP_ERR_INVALID, "sp_prop_set_path: unknown path");
+    p->path = path;
+    return SP_OK;
+}
+
+int sp_prop_set_tile(sp_prop* p, int tx, int ty, int nthreads, int kchunk)
+{
+    SP_CHECK(p, SP_ERR_INVALID, "sp_prop_set_tile: null propagator");
+    if (tx == 0) {
+        p->tile = FusedTile{0, 0, 0, kchunk > 0 ? kchunk : -1};
+        return SP_OK;
+    }
+    const FusedTile t{tx, ty, nthreads, kchunk};
+    SP_CHECK(fused_tile_supported(t, 1), SP_ERR_UNSUPPORTED, "sp_prop_set_tile: tile shape not compiled");
+    SP_CHECK(kchunk >= 0, SP_ERR_INVALID, "sp_prop_set_tile: negative kchunk");
+    p->tile = t;
+    return SP_OK;
+}
+
+static int rk4_common(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
+                      double tStart, double dt, int steps, bool f64_only, const char* name)
+{
+    SP_CHECK(p, SP_ERR_INVALID, std::string(name) + ": null propagator");
+    SP_TRY(check_prop_layout(p, lin, in, name));
+    SP_TRY(check_prop_layout(p, lout, out, name));
+    SP_CHECK(steps >= 0, SP_ERR_INVALID, std::string(name) + ": steps < 0");
+    SP_CHECK(std::isfinite(dt) && std::isfinite(tStart), SP_ERR_INVALID, std::string(name) + ": non-finite time");
+    if (f64_only)
+        SP_CHECK(lin->dtype == SP_F64 && lout->dtype == SP_F64, SP_ERR_INVALID,
+                 std::string(name) + ": fields must be SP_F64");
+    if (steps == 0) return SP_OK;  // Convection.cpp:296: out untouched
+    SP_HIP(hipSetDevice(p->ctx->device));
+    const View vin = view_of(*lin, in), vout = view_of(*lout, out);
+    const bool aliased = same_origin(lin, in, lout, out);
+    if (p->path == SP_PATH_STAGED && lin->dtype == SP_F64 && lout->dtype == SP_F64) {
+        SP_TRY(rk4_staged_step(p, vin, vout, tStart, dt));
+        for (int i = 1; i < steps; ++i) SP_TRY(rk4_staged_step(p, vout, vout, tStart + i * dt, dt));
+        return SP_OK;
+    }
+    return rk4_fused_run(p, lin->dtype, vin, lout->dtype, vout, aliased, tStart, dt, steps);
+}
+
+int sp_do_rk4(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out, double tStart,
+              double dt, int steps)
+{
+    return rk4_common(p, lin, in, lout, out, tStart, dt, steps, true, "sp_do_rk4");
+}
+
+int sp_do_rk4_ex(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out, double tStart,
+                 double dt, int steps)
+{
+    if (p && lin && lout && !(lin->dtype == SP_F64 && lout->dtype == SP_F64)) {
+        // mixed precision always runs the fused kernel
+        const int keep = p->path;
+        p->path = SP_PATH_FUSED;
+        const int r = rk4_common(p, lin, in, lout, out, tStart, dt, steps, false, "sp_do_rk4_ex");
+        p->path = keep;
+        return r;
+    }
+    return rk4_common(p, lin, in, lout, out, tStart, dt, steps, false, "sp_do_rk4_ex");
+}
+
+int sp_do_rk4_timestep(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out, double t,
+                       double dt)
+{
+    return rk4_common(p, lin, in, lout, out, t, dt, 1, true, "sp_do_rk4_timestep");
+}
+
+int sp_do_euler(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out, double tStart,
+                double dt, int steps)
+{
+    SP_CHECK(p, SP_ERR_INVALID, "sp_do_euler: null propagator");
+    SP_TRY(check_prop_layout(p, lin, in, "sp_do_euler"));
+    SP_TRY(check_prop_layout(p, lout, out, "sp_do_euler"));
+    SP_CHECK(lin->dtype == SP_F64 && lout->dtype == SP_F64, SP_ERR_INVALID, "sp_do_euler: fields must be SP_F64");
+    SP_CHECK(steps >= 0, SP_ERR_INVALID, "sp_do_euler: steps < 0");
+    if (steps == 0) return SP_OK;
+    SP_HIP(hipSetDevice(p->ctx->device));
+    const size_t bytes = (size_t)p->ni * p->nj * p->nk * 8;
+    const bool aliased = same_origin(lin, in, lout, out);
+    SP_TRY(p->scratch[0].ensure(bytes));
+    if (aliased) SP_TRY(p->scratch[1].ensure(bytes));
+    return euler_run(p, view_of(*lin, in), view_of(*lout, out), aliased, tStart, dt, steps, 1,
+                     cview(p->scratch[0].p, p->ni, p->nj), cview(p->scratch[1].p, p->ni, p->nj));
+}
+
+int sp_do_euler_timestep(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out, double t,
+                         double dt)
+{
+    return sp_do_euler(p, lin, in, lout, out, t, dt, 1);
+}
+
+int sp_periodic_fill(sp_ctx* c, const sp_layout* l, void* base)
+{
+    SP_CHECK(c && base, SP_ERR_INVALID, "sp_periodic_fill: null argument");
+    SP_TRY(check_layout(l, "sp_periodic_fill"));
+    SP_CHECK(l->dtype == SP_F64, SP_ERR_INVALID, "sp_periodic_fill: SP_F64 only");
+    SP_CHECK(l->halo >= 1, SP_ERR_INVALID, "sp_periodic_fill: layout has no halo");
+    SP_CHECK(l->ni >= l->halo && l->nj >= l->halo && l->nk >= l->halo, SP_ERR_INVALID,
+             "sp_periodic_fill: halo wider than the domain");
+    SP_HIP(hipSetDevice(c->device));
+    return launch_periodic_fill(c->stream, *l, base);
+}
+
+int sp_parareal_update(sp_ctx* c, const sp_layout* l, void* q, const void* fine, const void* cnew, const void* cold)
+{
+    SP_CHECK(c && q && fine && cnew && cold, SP_ERR_INVALID, "sp_parareal_update: null argument");
+    SP_TRY(check_layout(l, "sp_parareal_update"));
+    SP_CHECK(l->dtype == SP_F64, SP_ERR_INVALID, "sp_parareal_update: SP_F64 only");
+    SP_HIP(hipSetDevice(c->device));
+    return launch_update(c->stream, l->ni, l->nj, l->nk, view_of(*l, q), view_of(*l, fine), view_of(*l, cnew),
+                         view_of(*l, cold));
+}
+
+static int ensure_red(sp_ctx* c, int n)
+{
+    if (n <= c->red_cap) return SP_OK;
+    if (c->red) (void)hipFree(c->red);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    c->red = nullptr;
+    c->red_host = nullptr;
+    c->red_cap = 0;
+    if (hipMalloc(&c->red, n * sizeof(double)) != hipSuccess) return fail(SP_ERR_NOMEM, "reduction buffer");
+    if (hipHostMalloc(&c->red_host, n * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(SP_ERR_NOMEM, "pinned reduction buffer");
+    c->red_cap = n;
+    return SP_OK;
+}
+
+int sp_norms(sp_ctx* c, const sp_layout* la, const void* a, const sp_layout* lb, const void* b, double out[4])
+{
+    SP_CHECK(c && a && b && out, SP_ERR_INVALID, "sp_norms: null argument");
+    SP_TRY(check_layout(la, "sp_norms a"));
+    SP_TRY(check_layout(lb, "sp_norms b"));
+    SP_CHECK(la->dtype == SP_F64 && lb->dtype == SP_F64, SP_ERR_INVALID, "sp_norms: SP_F64 only");
+    SP_CHECK(la->ni == lb->ni && la->nj == lb->nj && la->nk == lb->nk, SP_ERR_INVALID, "sp_norms: size mismatch");
+    SP_HIP(hipSetDevice(c->device));
+    SP_TRY(ensure_red(c, kReduceBlocks * 4));
+    SP_TRY(launch_norms(c->stream, la->ni, la->nj, la->nk, view_of(*la, a), view_of(*lb, b), c->red, kReduceBlocks));
+    SP_HIP(hipMemcpyAsync(c->red_host, c->red, kReduceBlocks * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SP_HIP(hipStreamSynchronize(c->stream));
+    double r[4] = {0, 0, 0, 0};
+    for (int bI = 0; bI < kReduceBlocks; ++bI) {
+        r[0] = std::max(r[0], c->red_host[bI * 4 + 0]);
+        r[1] = std::max(r[1], c->red_host[bI * 4 + 1]);
+        r[2] += c->red_host[bI * 4 + 2];
+        r[3] += c->red_host[bI * 4 + 3];
+    }
+    for (int q = 0; q < 4; ++q) out[q] = r[q];
+    return SP_OK;
+}
+
+int sp_fill_exact(sp_ctx* c, const sp_layout* l, void* base, double nu0, double nufreq, double cx, double cy,
+                  double cz, double t)
+{
+    SP_CHECK(c && base, SP_ERR_INVALID, "sp_fill_exact: null argument");
+    SP_TRY(check_layout(l, "sp_fill_exact"));
+    SP_CHECK(l->dtype == SP_F64, SP_ERR_INVALID, "sp_fill_exact: SP_F64 only");
+    SP_HIP(hipSetDevice(c->device));
+    return launch_fill_exact(c->stream, *l, base, nu0, nufreq, cx, cy, cz, t);
+}
+
+int sp_fill_gaussians(sp_ctx* c, const sp_layout* l, void* base, int ng, const double* xc, const double* yc,
+                      const double* zc, const double* sigma, const double* amp)
+{
+    SP_CHECK(c && base && ng > 0 && xc && yc && zc && sigma && amp, SP_ERR_INVALID, "sp_fill_gaussians: bad argument");
+    SP_TRY(check_layout(l, "sp_fill_gaussians"));
+    SP_CHECK(l->dtype == SP_F64, SP_ERR_INVALID, "sp_fill_gaussians: SP_F64 only");
+    SP_HIP(hipSetDevice(c->device));
+    std::vector<double> g(5 * (size_t)ng);
+    for (int q = 0; q < ng; ++q) {
+        g[q] = xc[q];
+        g[ng + q] = yc[q];
+        g[2 * ng + q] = zc[q];
+        g[3 * ng + q] = sigma[q];
+        g[4 * ng + q] = amp[q];
+    }
+    double* gd = nullptr;
+    SP_HIP(hipMalloc(&gd, g.size() * sizeof(double)));
+    int r = SP_OK;
+    if (hipMemcpyAsync(gd, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        r = fail(SP_ERR_HIP, "sp_fill_gaussians: copy failed");
+    if (r == SP_OK) r = launch_fill_gaussians(c->stream, *l, base, ng, gd);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(gd);
+    return r;
+}
+
+}  // extern "C"
+
+// ======================================================================================
+// Batched Parareal.
+struct sp_parareal {
+    sp_prop* p = nullptr;
+    sp_parareal_cfg cfg{};
+    int S = 0, tsF = 0, tsG = 0;  // per slice (tsG counts sub-steps)
+    double dtF = 0, dtG = 0;      // dtG is the sub-step
+    double T = 0;
+    int gdt = SP_F64;
+    size_t n = 0;                 // points per slice
+    Buf U, F, FS, Gold, Gw0, Gw1, Q, nuF, nuG, part;
+    std::vector<double> nuF_host, nuG_host;
+    bool nu_const = true;
+    bool begun = false;
+    int iters_done = 0;
+    FusedTile tileF{}, tileG1{}, tileGS{};
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<double> part_host;
+};
+
+namespace {
+
+double slice_start(const sp_parareal* pr, int r) { return (pr->T / pr->S) * r; }  // main.cpp:157
+
+View batch_view(void* p, const sp_parareal* pr, int dt, int slice0 = 0)
+{
+    View v = cview((char*)p + (size_t)slice0 * pr->n * dtype_size(dt), pr->p->ni, pr->p->nj);
+    v.slice = (long long)pr->n;
+    return v;
+}
+
+// Coarse propagation of `nslices` batched slices, tsG sub-steps from per-slice start
+// times; step i of slice r runs at t0[r] + i*dtG (or at (first + i)*dtG from t = 0 for
+// the initial chain, first_index >= 0).  Input f64 (U) or gdt; output gdt in `dst`.
+int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View dst, View scr, const double* nu_dev_base,
+               long long first_index, int steps)
+{
+    sp_prop* p = pr->p;
+    hipStream_t s = p->ctx->stream;
+    if (p->path == SP_PATH_STAGED && pr->cfg.coarse_kind == SP_COARSE_RK4 && pr->gdt == SP_F64 && idt == SP_F64) {
+        // reference-order sweeps, slice by slice (src -> dst, then dst -> dst as DoRK4 does)
+        for (int q = 0; q < nslices; ++q) {
+            View a = src, b = dst;
+            a.p = (char*)src.p + (size_t)q * src.slice * 8;
+            b.p = (char*)dst.p + (size_t)q * dst.slice * 8;
+            a.slice = b.slice = 0;
+            for (int i = 0; i < steps; ++i) {
+                const double t = first_index >= 0 ? 0. + (double)(first_index + i) * pr->dtG
+                                                  : slice_start(pr, slice0 + q) + i * pr->dtG;
+                SP_TRY(rk4_staged_step(p, i == 0 ? a : b, b, t, pr->dtG));
+            }
+        }
+        return SP_OK;
+    }
+    View cur = src;
+    int cdt = idt;
+    for (int i = 0; i < steps; ++i) {
+        const int left = steps - 1 - i;
+        const View d = (left % 2 == 0) ? dst : scr;
+        double nu;
+        const double* nud = nullptr;
+        if (first_index >= 0) {
+            nu = computenu(p, 0. + (double)(first_index + i) * pr->dtG);
+        } else if (pr->nu_const) {
+            nu = p->nu0;
+        } else if (nslices == 1) {
+            nu = pr->nuG_host[(size_t)i * pr->S + slice0];
+        } else {
+            nu = p->nu0;
+            nud = nu_dev_base + (size_t)i * pr->S + slice0;
+        }
+        if (pr->cfg.coarse_kind == SP_COARSE_RK4) {
+            const FusedTile& tile = nslices == 1 ? pr->tileG1 : pr->tileGS;
+            SP_TRY(launch_rk4_fused(s, fused_step(p, nslices, pr->dtG, nud, nu), tile, cdt, cur, pr->gdt, d));
+        } else {
+            if (nud) return fail(SP_ERR_UNSUPPORTED, "batched Euler coarse with time-dependent nu");
+            const StagedParams P{p->ni, p->nj, p->nk, p->cx, p->cy, p->cz, p->dx, p->dx * p->dx};
+            SP_TRY(launch_euler_fused(s, P, cur, d, nu, pr->dtG, p->upwind, nslices));
+        }
+        cur = d;
+        cdt = pr->gdt;
+    }
+    return SP_OK;
+}
+
+double fine_time(const sp_parareal* pr, int r, int i)
+{
+    const double t0 = slice_start(pr, r);
+    // Parareal.h:119-128: the async schedule runs step 0 at tStart, then from tStart + dtF
+    if (pr->cfg.async_schedule && r > 0 && i > 0) return (t0 + pr->dtF) + (i - 1) * pr->dtF;
+    return t0 + i * pr->dtF;
+}
+
+int fine_sweep(sp_parareal* pr)
+{
+    sp_prop* p = pr->p;
+    hipStream_t s = p->ctx->stream;
+    if (p->path == SP_PATH_STAGED) {
+        // reference-order sweeps, slice by slice: DoRK4(U_r -> F_r, tStart_r, dtF, tsF)
+        for (int r = 0; r < pr->S; ++r) {
+            const View Ur = cview((char*)pr->U.p + (size_t)r * pr->n * 8, p->ni, p->nj);
+            const View Fr = cview((char*)pr->F.p + (size_t)r * pr->n * 8, p->ni, p->nj);
+            for (int i = 0; i < pr->tsF; ++i) SP_TRY(rk4_staged_step(p, i == 0 ? Ur : Fr, Fr, fine_time(pr, r, i), pr->dtF));
+        }
+        return SP_OK;
+    }
+    const View U = batch_view(pr->U.p, pr, SP_F64), F = batch_view(pr->F.p, pr, SP_F64),
+               FS = batch_view(pr->FS.p, pr, SP_F64);
+    View cur = U;
+    for (int i = 0; i < pr->tsF; ++i) {
+        const int left = pr->tsF - 1 - i;
+        const View d = (left % 2 == 0) ? F : FS;
+        const double* nud = pr->nu_const ? nullptr : (const double*)pr->nuF.p + (size_t)i * pr->S;
+        SP_TRY(launch_rk4_fused(s, fused_step(p, pr->S, pr->dtF, nud, p->nu0), pr->tileF, SP_F64, cur, SP_F64, d));
+        cur = d;
+    }
+    return SP_OK;
+}
+
+int coarse_chain(sp_parareal* pr)
+{
+    sp_prop* p = pr->p;
+    hipStream_t s = p->ctx->stream;
+    const int gs = dtype_size(pr->gdt);
+    for (int r = 0; r < pr->S; ++r) {
+        View cold = cview((char*)pr->Gold.p + (size_t)r * pr->n * gs, p->ni, p->nj);
+        View cnew;
+        if (r == 0) {
+            // Rank 0 never receives: G(U_0) is recomputed from the same input and times, so it
+            // equals its initial coarse value bit for bit.
+            cnew = cold;
+        } else {
+            const View Ur = cview((char*)pr->U.p + (size_t)r * pr->n * 8, p->ni, p->nj);
+            const View A = cview(pr->Gw0.p, p->ni, p->nj), B = cview(pr->Gw1.p, p->ni, p->nj);
+            SP_TRY(coarse_run(pr, 1, r, SP_F64, Ur, A, B, (const double*)pr->nuG.p, -1, pr->tsG));
+            cnew = A;
+        }
+        const View fine = cview((char*)pr->F.p + (size_t)r * pr->n * 8, p->ni, p->nj);
+        const View dst = (r < pr->S - 1) ? cview((char*)pr->U.p + (size_t)(r + 1) * pr->n * 8, p->ni, p->nj)
+                                         : cview(pr->Q.p, p->ni, p->nj);
+        SP_TRY(launch_correction(s, p->ni, p->nj, p->nk, pr->gdt, cnew, cold, fine, dst,
+                                 (double*)pr->part.p + (size_t)r * kReduceBlocks * 2, kReduceBlocks));
+    }
+    return SP_OK;
+}
+
+double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out)
+{
+    SP_CHECK(p && cfg && out, SP_ERR_INVALID, "sp_parareal_create: null argument");
+    SP_CHECK(cfg->slices >= 1, SP_ERR_INVALID, "sp_parareal_create: slices < 1");
+    SP_CHECK(cfg->kmax >= 0 && cfg->kmax <= SP_MAX_ITERS, SP_ERR_INVALID, "sp_parareal_create: kmax out of range");
+    SP_CHECK(cfg->end_time > 0 && std::isfinite(cfg->end_time), SP_ERR_INVALID, "sp_parareal_create: end_time");
+    SP_CHECK(cfg->steps_fine >= 1 && cfg->steps_coarse >= 1, SP_ERR_INVALID, "sp_parareal_create: step counts");
+    SP_CHECK(cfg->coarse_kind == SP_COARSE_EULER || cfg->coarse_kind == SP_COARSE_RK4, SP_ERR_INVALID,
+             "sp_parareal_create: coarse_kind");
+    SP_CHECK(cfg->coarse_dtype == SP_F64 || cfg->coarse_dtype == SP_F32 || cfg->coarse_dtype == SP_BF16,
+             SP_ERR_INVALID, "sp_parareal_create: coarse_dtype");
+    SP_CHECK(cfg->coarse_kind == SP_COARSE_RK4 || cfg->coarse_dtype == SP_F64, SP_ERR_UNSUPPORTED,
+             "sp_parareal_create: Euler coarse propagator is SP_F64 only");
+    SP_CHECK(cfg->coarse_substeps >= 1, SP_ERR_INVALID, "sp_parareal_create: coarse_substeps < 1");
+    SP_CHECK(cfg->tol >= 0, SP_ERR_INVALID, "sp_parareal_create: tol < 0");
+    SP_HIP(hipSetDevice(p->ctx->device));
+    sp_parareal* pr = new sp_parareal;
+    pr->p = p;
+    pr->cfg = *cfg;
+    pr->S = cfg->slices;
+    pr->T = cfg->end_time;
+    // RuntimeConfiguration.cpp:259-277: totals rounded up to multiples of the slice count
+    int sf = cfg->steps_fine, sc = cfg->steps_coarse;
+    if (sf % pr->S != 0) sf = (sf / pr->S + 1) * pr->S;
+    if (sc % pr->S != 0) sc = (sc / pr->S + 1) * pr->S;
+    pr->dtF = pr->T / sf;  // RuntimeConfiguration.h:62-73
+    pr->tsF = sf / pr->S;
+    pr->dtG = (pr->T / sc) / cfg->coarse_substeps;
+    pr->tsG = (sc / pr->S) * cfg->coarse_substeps;
+    pr->gdt = cfg->coarse_dtype;
+    pr->n = (size_t)p->ni * p->nj * p->nk;
+    pr->nu_const = (p->nufreq == 0.);
+    const size_t nb = pr->n * pr->S;
+    int r = SP_OK;
+    auto chk = [&](int x) {
+        if (r == SP_OK && x != SP_OK) r = x;
+    };
+    chk(pr->U.ensure(nb * 8));
+    chk(pr->F.ensure(nb * 8));
+    chk(pr->FS.ensure(nb * 8));
+    chk(pr->Gold.ensure(nb * dtype_size(pr->gdt)));
+    chk(pr->Gw0.ensure(pr->n * dtype_size(pr->gdt)));
+    chk(pr->Gw1.ensure(pr->n * dtype_size(pr->gdt)));
+    chk(pr->Q.ensure(pr->n * 8));
+    chk(pr->part.ensure((size_t)pr->S * kReduceBlocks * 2 * sizeof(double)));
+    if (r != SP_OK) {
+        sp_parareal_destroy(pr);
+        return r;
+    }
+    // Per-slice nu tables (computenu at every step start, Convection.cpp:321).
+    if (!pr->nu_const) {
+        pr->nuF_host.resize((size_t)pr->tsF * pr->S);
+        pr->nuG_host.resize((size_t)pr->tsG * pr->S);
+        for (int rr = 0; rr < pr->S; ++rr) {
+            const double t0 = slice_start(pr, rr);
+            for (int i = 0; i < pr->tsF; ++i) pr->nuF_host[(size_t)i * pr->S + rr] = computenu(p, fine_time(pr, rr, i));
+            for (int i = 0; i < pr->tsG; ++i) pr->nuG_host[(size_t)i * pr->S + rr] = computenu(p, t0 + i * pr->dtG);
+        }
+        chk(pr->nuF.ensure(pr->nuF_host.size() * 8));
+        chk(pr->nuG.ensure(pr->nuG_host.size() * 8));
+        if (r == SP_OK && (hipMemcpy(pr->nuF.p, pr->nuF_host.data(), pr->nuF_host.size() * 8, hipMemcpyHostToDevice) !=
+                               hipSuccess ||
+                           hipMemcpy(pr->nuG.p, pr->nuG_host.data(), pr->nuG_host.size() * 8, hipMemcpyHostToDevice) !=
+                               hipSuccess))
+            r = fail(SP_ERR_HIP, "sp_parareal_create: nu table upload");
+        if (r != SP_OK) {
+            sp_parareal_destroy(pr);
+            return r;
+        }
+    }
+    if (p->path == SP_PATH_STAGED) {
+        // staged temporaries exist before any graph capture
+        const size_t bytes = pr->n * 8;
+        for (Buf* b : {&p->qI, &p->k1, &p->k2, &p->k3, &p->k4}) chk(b->ensure(bytes));
+        if (r != SP_OK) {
+            sp_parareal_destroy(pr);
+            return r;
+        }
+    }
+    pr->tileF = pick_tile(p, pr->S, 1);
+    pr->tileG1 = pick_tile(p, 1, pr->gdt == SP_F64);
+    pr->tileGS = pick_tile(p, pr->S, pr->gdt == SP_F64);
+    pr->part_host.resize((size_t)pr->S * kReduceBlocks * 2);
+    *out = pr;
+    return SP_OK;
+}
+
+int sp_parareal_destroy(sp_parareal* pr)
+{
+    if (!pr) return SP_OK;
+    (void)hipStreamSynchronize(pr->p->ctx->stream);
+    if (pr->gexec) (void)hipGraphExecDestroy(pr->gexec);
+    if (pr->graph) (void)hipGraphDestroy(pr->graph);
+    for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->nuF, &pr->nuG, &pr->part})
+        b->release();
+    delete pr;
+    return SP_OK;
+}
+
+int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
+{
+    SP_CHECK(pr, SP_ERR_INVALID, "sp_parareal_begin: null handle");
+    sp_prop* p = pr->p;
+    SP_TRY(check_prop_layout(p, l0, u0, "sp_parareal_begin"));
+    SP_CHECK(l0->dtype == SP_F64, SP_ERR_INVALID, "sp_parareal_begin: u0 must be SP_F64");
+    SP_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    const int gs = dtype_size(pr->gdt);
+    // U_0 = u0
+    SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, SP_F64, view_of(*l0, u0), SP_F64, cview(pr->U.p, p->ni, p->nj)));
+    // U_r = G^{r*tsG}(u0), one continuous coarse run from t = 0 (Parareal.h:113): rank r's own
+    // run visits the same states at the same times (step j at t = 0 + j*dtG), so its value is
+    // the snapshot after r*tsG steps.  The chain state stays in coarse precision and
+    // alternates between Gw0 and Gw1; slice 0 of Gold (filled below) is the sub-step scratch.
+    {
+        const View bufs[2] = {cview(pr->Gw0.p, p->ni, p->nj), cview(pr->Gw1.p, p->ni, p->nj)};
+        const View Z = cview(pr->Gold.p, p->ni, p->nj);
+        int cur = -1;
+        for (int r = 1; r < pr->S; ++r) {
+            const int nxt = cur < 0 ? 0 : 1 - cur;
+            const View src = cur < 0 ? cview(pr->U.p, p->ni, p->nj) : bufs[cur];
+            SP_TRY(coarse_run(pr, 1, r - 1, cur < 0 ? SP_F64 : pr->gdt, src, bufs[nxt], Z, nullptr,
+                              (long long)(r - 1) * pr->tsG, pr->tsG));
+            cur = nxt;
+            SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt, bufs[cur], SP_F64,
+                                  cview((char*)pr->U.p + (size_t)r * pr->n * 8, p->ni, p->nj)));
+        }
+    }
+    // Gold_r = G(U_r) from tStart_r (Parareal.h:114), all slices in one batch; FS is scratch.
+    {
+        View src = batch_view(pr->U.p, pr, SP_F64);
+        View dst = batch_view(pr->Gold.p, pr, pr->gdt);
+        View scr = batch_view(pr->FS.p, pr, pr->gdt);
+        if (pr->cfg.coarse_kind == SP_COARSE_EULER && !pr->nu_const) {
+            for (int r = 0; r < pr->S; ++r) {
+                View sr = cview((char*)pr->U.p + (size_t)r * pr->n * 8, p->ni, p->nj);
+                View dr = cview((char*)pr->Gold.p + (size_t)r * pr->n * gs, p->ni, p->nj);
+                SP_TRY(coarse_run(pr, 1, r, SP_F64, sr, dr, cview(pr->Gw0.p, p->ni, p->nj), nullptr, -1, pr->tsG));
+            }
+        } else {
+            SP_TRY(coarse_run(pr, pr->S, 0, SP_F64, src, dst, scr, (const double*)pr->nuG.p, -1, pr->tsG));
+        }
+    }
+    // The coarse prediction of the final state is the last slice's coarse value.
+    SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt,
+                          cview((char*)pr->Gold.p + (size_t)(pr->S - 1) * pr->n * gs, p->ni, p->nj), SP_F64,
+                          cview(pr->Q.p, p->ni, p->nj)));
+    SP_HIP(hipStreamSynchronize(s));
+    pr->begun = true;
+    pr->iters_done = 0;
+    return SP_OK;
+}
+
+static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tcoarse)
+{
+    sp_prop* p = pr->p;
+    hipStream_t s = p->ctx->stream;
+    const double a = now();
+    if (pr->cfg.use_graphs) {
+        // Capture needs a created stream: the null stream is first drained, then the graph
+        // runs on the context's own stream.
+        hipStream_t user = s;
+        if (user == nullptr) {
+            SP_HIP(hipStreamSynchronize(nullptr));
+            s = p->ctx->own;
+        }
+        if (!pr->gexec) {
+            p->ctx->stream = s;
+            SP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int r = fine_sweep(pr);
+            if (r == SP_OK) r = coarse_chain(pr);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            p->ctx->stream = user;
+            if (r != SP_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            SP_HIP(e);
+            pr->graph = g;
+            SP_HIP(hipGraphInstantiate(&pr->gexec, g, nullptr, nullptr, 0));
+        }
+        SP_HIP(hipGraphLaunch(pr->gexec, s));
+        SP_HIP(hipStreamSynchronize(s));
+        if (tfine) *tfine = now() - a;
+        if (tcoarse) *tcoarse = 0.;
+    } else {
+        SP_TRY(fine_sweep(pr));
+        SP_HIP(hipStreamSynchronize(s));
+        const double b = now();
+        SP_TRY(coarse_chain(pr));
+        SP_HIP(hipStreamSynchronize(s));
+        if (tfine) *tfine = b - a;
+        if (tcoarse) *tcoarse = now() - b;
+    }
+    SP_HIP(hipMemcpy(pr->part_host.data(), pr->part.p, pr->part_host.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double d2 = 0., q2 = 0.;
+    for (size_t i = 0; i < pr->part_host.size(); i += 2) {
+        d2 += pr->part_host[i];
+        q2 += pr->part_host[i + 1];
+    }
+    const double inc = std::sqrt(d2) / std::sqrt(q2);
+    pr->iters_done++;
+    if (incr) *incr = inc;
+    // A non-finite increment, or a change a million times the size of the state itself,
+    // means an unstable propagator: report it instead of returning the data.
+    SP_CHECK(std::isfinite(inc) && inc <= 1e6, SP_ERR_DIVERGED,
+             "Parareal increment " + std::to_string(inc) + ": coarse or fine step unstable");
+    return SP_OK;
+}
+
+int sp_parareal_iterate(sp_parareal* pr, double* increment)
+{
+    SP_CHECK(pr, SP_ERR_INVALID, "sp_parareal_iterate: null handle");
+    SP_CHECK(pr->begun, SP_ERR_INVALID, "sp_parareal_iterate: call sp_parareal_begin first");
+    SP_HIP(hipSetDevice(pr->p->ctx->device));
+    return iterate_impl(pr, increment, nullptr, nullptr);
+}
+
+int sp_parareal_solution(sp_parareal* pr, const sp_layout* lq, void* q)
+{
+    SP_CHECK(pr, SP_ERR_INVALID, "sp_parareal_solution: null handle");
+    sp_prop* p = pr->p;
+    SP_TRY(check_prop_layout(p, lq, q, "sp_parareal_solution"));
+    SP_CHECK(lq->dtype == SP_F64, SP_ERR_INVALID, "sp_parareal_solution: q must be SP_F64");
+    SP_CHECK(pr->begun, SP_ERR_INVALID, "sp_parareal_solution: call sp_parareal_begin first");
+    SP_HIP(hipSetDevice(p->ctx->device));
+    return launch_convert(p->ctx->stream, p->ni, p->nj, p->nk, 1, SP_F64, cview(pr->Q.p, p->ni, p->nj), SP_F64,
+                          view_of(*lq, q));
+}
+
+int sp_parareal_run(sp_parareal* pr, const sp_layout* l0, const void* u0, const sp_layout* lq, void* q,
+                    sp_parareal_report* rep)
+{
+    SP_CHECK(pr, SP_ERR_INVALID, "sp_parareal_run: null handle");
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->steps_fine_per_slice = pr->tsF;
+        rep->steps_coarse_per_slice = pr->tsG;
+        rep->dt_fine = pr->dtF;
+        rep->dt_coarse = pr->dtG;
+    }
+    const double t0 = now();
+    SP_TRY(sp_parareal_begin(pr, l0, u0));
+    const double t1 = now();
+    if (rep) rep->t_init = t1 - t0;
+    int k = 0;
+    for (; k < pr->cfg.kmax; ++k) {
+        double inc = 0., tf = 0., tc = 0.;
+        SP_TRY(iterate_impl(pr, &inc, &tf, &tc));
+        if (rep) {
+            rep->increment[k] = inc;
+            rep->t_fine[k] = tf;
+            rep->t_coarse[k] = tc;
+            rep->iterations = k + 1;
+        }
+        if (pr->cfg.tol > 0 && inc <= pr->cfg.tol) {
+            ++k;
+            break;
+        }
+    }
+    if (q) SP_TRY(sp_parareal_solution(pr, lq, q));
+    SP_HIP(hipStreamSynchronize(pr->p->ctx->stream));
+    if (rep) rep->t_total = now() - t0;
+    return SP_OK;
+}
+
+int sp_parareal_serial(sp_parareal* pr, const sp_layout* l0, const void* u0, const sp_layout* lq, void* q)
+{
+    SP_CHECK(pr, SP_ERR_INVALID, "sp_parareal_serial: null handle");
+    // Parareal.h:250-253: DoRK4(qinitial, q, 0, dtFine, tsFine*S)
+    return sp_do_rk4(pr->p, l0, u0, lq, q, 0., pr->dtF, pr->tsF * pr->S);
+}
+
+}  // extern "C"

@@ -1,0 +1,298 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Bars:
+  * reference-order paths (staged RK4 sweeps, Euler, periodic fill, update): bit-for-bit;
+  * fused RK4 kernel (nested-stage form, FMA): max |gpu - oracle| / max |oracle| <= 1e-12
+    (the north-star tolerance for the fine propagator), typically ~1e-15;
+  * coarse propagator in f32 / bf16 storage: checked against the f64 oracle with the
+    tolerance of its storage precision, and through Parareal's convergence to the f64
+    serial fine solution.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle as O  # noqa: E402
+from stencilparareal_amd.config import RuntimeConfiguration  # noqa: E402
+
+C3 = (1.0, -0.5, 0.25)
+
+
+@pytest.fixture(scope="module")
+def sp():
+    import stencilparareal_amd as spm
+    from stencilparareal_amd import field
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    field.Context.default(0)
+    return spm
+
+
+def rand_field(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape)
+
+
+def smooth_field(N, seed):
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    q = np.zeros((N, N, N))
+    for _ in range(4):
+        m = rng.integers(0, 4, 3)
+        ph = rng.uniform(0, 2 * np.pi)
+        q += rng.uniform(0.5, 1) * np.cos(2 * np.pi * (m[0] * i + m[1] * j + m[2] * k) / N + ph)
+    return q
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("kind_in,kind_out", [("compact", "compact"), ("stella", "stella"), ("stella", "compact")])
+def test_staged_rk4_bitwise(sp, kind_in, kind_out):
+    N, nu, dt, steps = 12, 0.01, 1e-3, 4
+    q0 = rand_field((N, N, N), 1)
+    conv = sp.Convection(N, N, N, 1. / N, nu, 0.0, *C3, path="staged")
+    a, b = conv.field(kind=kind_in), conv.field(kind=kind_out)
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, dt, steps)
+    oc = O.OConvection(N, N, N, 1. / N, nu, 0.0, *C3)
+    assert np.array_equal(b.get(), oc.rk4(q0, 0.0, dt, steps))
+
+
+def test_staged_rk4_bitwise_time_dependent_nu_in_place(sp):
+    N, nu, dt, steps = 10, 0.02, 2e-3, 3
+    q0 = rand_field((N, N, N), 2)
+    conv = sp.Convection(N, N, N, 1. / N, nu, 50.0, *C3, path="staged")
+    a = conv.field()
+    a.set(q0)
+    conv.DoRK4(a, a, 0.3, dt, steps)
+    oc = O.OConvection(N, N, N, 1. / N, nu, 50.0, *C3)
+    assert np.array_equal(a.get(), oc.rk4(q0, 0.3, dt, steps))
+
+
+@pytest.mark.parametrize("upwind", [True, False])
+@pytest.mark.parametrize("cvec", [C3, (-1.0, 0.5, -0.25)])
+def test_euler_bitwise(sp, upwind, cvec):
+    N, nu, dt, steps = 12, 0.01, 5e-4, 5
+    q0 = rand_field((N, N, N), 3)
+    conv = sp.Convection(N, N, N, 1. / N, nu, 0.0, *cvec, upwind=upwind)
+    a, b = conv.field(), conv.field(kind="stella")
+    a.set(q0)
+    conv.DoEuler(a, b, 0.0, dt, steps)
+    oc = O.OConvection(N, N, N, 1. / N, nu, 0.0, *cvec, upwind=upwind)
+    assert np.array_equal(b.get(), oc.euler(q0, 0.0, dt, steps))
+    # in place, one step (exercises the scratch copy-back)
+    conv.DoEuler(a, a, 0.0, dt, 1)
+    assert np.array_equal(a.get(), oc.euler(q0, 0.0, dt, 1))
+
+
+@pytest.mark.parametrize("N", [8, 16, 20, 32, 48, 64])
+@pytest.mark.parametrize("tile", [(32, 32, 1024), (24, 24, 512), (32, 16, 512), (16, 16, 256)])
+def test_fused_rk4_vs_oracle(sp, N, tile):
+    nu, dt, steps = 0.01, 1e-3 * (16 / N) ** 2, 6
+    q0 = smooth_field(N, N)
+    conv = sp.Convection(N, N, N, 1. / N, nu, 0.0, *C3)
+    conv.set_tile(*tile, kchunk=0 if N < 32 else N // 2)
+    a, b = conv.field(), conv.field(kind="stella")
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, dt, steps)
+    ref = O.OConvection(N, N, N, 1. / N, nu, 0.0, *C3).rk4(q0, 0.0, dt, steps)
+    assert relerr(b.get(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("kchunk", [0, 7, 16])
+def test_fused_rk4_kchunks_and_nonsquare(sp, kchunk):
+    ni, nj, nk = 40, 24, 36
+    q0 = rand_field((nk, nj, ni), 5)
+    conv = sp.Convection(ni, nj, nk, 1. / 32, 0.01, 0.0, *C3)
+    conv.set_tile(32, 32, 1024, kchunk)
+    a, b = conv.field(), conv.field()
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, 2e-4, 3)
+    ref = O.OConvection(ni, nj, nk, 1. / 32, 0.01, 0.0, *C3).rk4(q0, 0.0, 2e-4, 3)
+    assert relerr(b.get(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("steps", [0, 1, 2, 3])
+def test_fused_rk4_aliasing_and_zero_steps(sp, steps):
+    N = 16
+    q0 = smooth_field(N, 7)
+    conv = sp.Convection(N, N, N, 1. / N, 0.01, 30.0, *C3)
+    a = conv.field()
+    a.set(q0)
+    conv.DoRK4(a, a, 0.1, 1e-3, steps)
+    ref = O.OConvection(N, N, N, 1. / N, 0.01, 30.0, *C3).rk4(q0, 0.1, 1e-3, steps) if steps else q0
+    assert relerr(a.get(), ref) <= 1e-13
+
+
+def test_fused_rk4_long_run(sp):
+    N, steps = 32, 300
+    q0 = O.fill_gaussians(N, [0.3, 0.7], [0.5, 0.2], [0.6, 0.9], [0.08, 0.05], [1.0, 0.7])
+    conv = sp.Convection(N, N, N, 1. / N, 1e-2, 0.0, 1.0, 0.5, 0.25)
+    a, b = conv.field(), conv.field()
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, 1e-4, steps)
+    ref = O.OConvection(N, N, N, 1. / N, 1e-2, 0.0, 1.0, 0.5, 0.25).rk4(q0, 0.0, 1e-4, steps)
+    assert relerr(b.get(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [("f32", 2e-6), ("bf16", 2e-2)])
+def test_reduced_precision_rk4_against_f64_oracle(sp, dtype, tol):
+    N, steps = 32, 20
+    q0 = smooth_field(N, 11)
+    conv = sp.Convection(N, N, N, 1. / N, 5e-3, 0.0, *C3)
+    a, b = conv.field(), conv.field(dtype=dtype)
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, 1e-3, steps)
+    ref = O.OConvection(N, N, N, 1. / N, 5e-3, 0.0, *C3).rk4(q0, 0.0, 1e-3, steps)
+    err = relerr(b.get(), ref)
+    assert err <= tol
+    assert err > 1e-14  # it really ran in reduced precision
+
+
+def test_periodic_fill_bitwise(sp):
+    N = 6
+    f = sp.ConvectionField("q", N, N + 1, N - 1, kind="stella")
+    data = torch.randn(f.storage.numel(), dtype=torch.float64, device="cuda")
+    f.storage.copy_(data)
+    sp.periodic_fill(f)
+    host = data.cpu().numpy().copy()
+    lay = f.layout
+    base = host[lay.origin - (2 + 2 * lay.sj + 2 * lay.sk):]
+    base = np.ascontiguousarray(base)
+    O.apply_periodic(base, lay.ni, lay.nj, lay.nk, (2, 2, 2, 2, 2, 2), (1, lay.sj, lay.sk))
+    got = f.storage.cpu().numpy()[lay.origin - (2 + 2 * lay.sj + 2 * lay.sk):]
+    assert np.array_equal(got, base)
+
+
+def test_update_and_norms(sp):
+    from stencilparareal_amd import _lib as L
+    from stencilparareal_amd.field import Context
+    N = 16
+    ctx = Context.default()
+    fs = [sp.ConvectionField(n, N, N, N, kind="stella") for n in ("q", "f", "cn", "co")]
+    vals = [rand_field((N, N, N), 20 + i) for i in range(4)]
+    for f, v in zip(fs, vals):
+        f.set(v)
+    ctx.sync_stream()
+    L.call("sp_parareal_update", ctx.h, fs[0].lp(), fs[0].vp(), fs[1].vp(), fs[2].vp(), fs[3].vp())
+    assert np.array_equal(fs[0].get(), vals[2] + vals[1] - vals[3])
+    e = sp.error_reference(fs[1], fs[2])
+    assert e == O.error_reference(vals[1], vals[2])
+    assert abs(sp.rel_l2(fs[1], fs[2]) - O.rel_l2(vals[1], vals[2])) < 1e-14
+
+
+def test_initial_conditions(sp):
+    N = 24
+    f = sp.ConvectionField("q", N, N, N)
+    sp.fill_exact(f, 0.1, 0.0, 1.0, 1.0, 1.0, 0.01)
+    assert np.abs(f.get() - O.fill_exact(N, 0.1, 0.0, 1.0, 1.0, 1.0, 0.01)).max() < 1e-15
+    args = ([0.05, 0.5], [0.5, 0.9], [0.5, 0.1], [0.1, 0.05], [2.0, 0.6])
+    sp.fill_gaussians(f, *args)
+    assert np.abs(f.get() - O.fill_gaussians(N, *args)).max() < 1e-14
+
+
+def test_invalid_arguments_return_errors(sp):
+    from stencilparareal_amd._lib import SPError
+    conv = sp.Convection(8, 8, 8, 1. / 8, 0.01, 0.0, *C3)
+    a, b = conv.field(), conv.field()
+    with pytest.raises(SPError) as ei:
+        conv.DoRK4(a, b, 0.0, 1e-3, -1)
+    assert ei.value.code == 1
+    wrong = sp.ConvectionField("w", 8, 8, 9)
+    with pytest.raises(SPError):
+        conv.DoRK4(a, wrong, 0.0, 1e-3, 1)
+    with pytest.raises(SPError):
+        conv.DoEuler(a, conv.field(dtype="f32"), 0.0, 1e-3, 1)
+
+
+# ---------------------------------------------------------------------------------------------
+# Parareal
+
+def _parareal_case(sp, path, coarse, N=12, S=4, kmax=4, nufreq=0.0):
+    conf = RuntimeConfiguration(gridSize=N, endTime=0.02, timeStepsFine=64, timeStepsCoarse=8, nu0=0.05,
+                                nufreq=nufreq, cx=1.0, cy=-0.5, cz=0.25, kmax=kmax, async_=False,
+                                timeSlices=S)
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, path=path)
+    u0 = O.fill_gaussians(N, [0.3, 0.6], [0.4, 0.7], [0.5, 0.2], [0.1, 0.08], [1.0, 0.5])
+    oc = O.OConvection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz)
+    _, iters = oc.parareal(u0, S, kmax, conf.endTime, conf.dtFine(), conf.timeStepsFinePerTimeSlice(),
+                           conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(), coarse)
+    serial = oc.serial(u0, conf.dtFine(), conf.timeStepsFine)
+    return conf, conv, u0, iters, serial
+
+
+@pytest.mark.parametrize("coarse", ["euler", "rk4"])
+@pytest.mark.parametrize("nufreq", [0.0, 80.0])
+def test_parareal_staged_bitwise_every_iterate(sp, coarse, nufreq):
+    conf, conv, u0, iters, _ = _parareal_case(sp, "staged", coarse, nufreq=nufreq)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse=coarse)
+    pr.begin()
+    assert np.array_equal(pr.solution().get(), iters[0])
+    for k in range(1, conf.kmax + 1):
+        pr.iterate()
+        assert np.array_equal(pr.solution().get(), iters[k]), k
+
+
+@pytest.mark.parametrize("coarse", ["euler", "rk4"])
+@pytest.mark.parametrize("graphs", [False, True])
+def test_parareal_fused_vs_oracle(sp, coarse, graphs):
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", coarse)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse=coarse, use_graphs=graphs)
+    rep = pr.DoParallel()
+    assert rep.iterations == conf.kmax
+    assert relerr(sol.get(), iters[conf.kmax]) <= 1e-12
+    # DoSerial
+    pr.DoSerial()
+    assert relerr(sol.get(), serial) <= 1e-12
+
+
+@pytest.mark.parametrize("gdt", ["f32", "bf16"])
+def test_parareal_mixed_precision_coarse_reaches_serial_fine(sp, gdt):
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4", S=4, kmax=4)
+    init, sol, ref = conv.field(), conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=gdt)
+    pr.begin()
+    errs = [O.error_reference(pr.solution().get(), serial)]
+    for _ in range(conf.kmax):
+        pr.iterate()
+        errs.append(O.error_reference(pr.solution().get(), serial))
+    assert errs[-1] <= 1e-13          # k = S: the serial fine solution, whatever G is
+    assert errs[1] < errs[0]
+
+
+def test_parareal_tolerance_stops_early(sp):
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4", S=4, kmax=4)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", tol=1e-6)
+    rep = pr.DoParallel()
+    assert 1 <= rep.iterations < conf.kmax
+    assert rep.increment[rep.iterations - 1] <= 1e-6
+
+
+def test_parareal_diverges_loudly(sp):
+    # a coarse step far beyond the RK4 stability limit blows up: reported, not returned
+    from stencilparareal_amd._lib import SPError
+    N, S = 16, 4
+    conf = RuntimeConfiguration(gridSize=N, endTime=2.0, timeStepsFine=4000, timeStepsCoarse=4, nu0=0.5,
+                                cx=1.0, cy=1.0, cz=1.0, kmax=3, async_=False, timeSlices=S)
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, 1.0, 1.0, 1.0)
+    init, sol = conv.field(), conv.field()
+    init.set(smooth_field(N, 3))
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4")
+    pr.begin()
+    with pytest.raises(SPError) as ei:
+        for _ in range(3):
+            pr.iterate()
+    assert ei.value.code == 4
