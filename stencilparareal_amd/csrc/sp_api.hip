@@ -43,7 +43,10 @@ int check_layout(const sp_layout* l, const char* what)
 
 using namespace sp;
 
+// Handles are reference counted: a propagator holds its context, a Parareal driver holds
+// its propagator, so destroying them in any order (as garbage collectors do) is safe.
 struct sp_ctx {
+    int refs = 1;
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
@@ -74,6 +77,7 @@ struct Buf {
 };
 
 struct sp_prop {
+    int refs = 1;
     sp_ctx* ctx = nullptr;
     int ni = 0, nj = 0, nk = 0;
     double dx = 0, nu0 = 0, nufreq = 0, cx = 0, cy = 0, cz = 0;
@@ -255,15 +259,20 @@ int sp_ctx_create(int device, sp_ctx** out)
     return SP_OK;
 }
 
-int sp_ctx_destroy(sp_ctx* c)
+static void ctx_release(sp_ctx* c)
 {
-    if (!c) return SP_OK;
+    if (!c || --c->refs > 0) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->red) (void)hipFree(c->red);
     if (c->red_host) (void)hipHostFree(c->red_host);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
+}
+
+int sp_ctx_destroy(sp_ctx* c)
+{
+    ctx_release(c);
     return SP_OK;
 }
 
@@ -357,6 +366,7 @@ int sp_prop_create(sp_ctx* c, int isize, int jsize, int ksize, double dx, double
     SP_CHECK(dx > 0 && std::isfinite(dx), SP_ERR_INVALID, "sp_prop_create: dx must be positive");
     sp_prop* p = new sp_prop;
     p->ctx = c;
+    c->refs++;
     p->ni = isize;
     p->nj = jsize;
     p->nk = ksize;
@@ -370,12 +380,20 @@ int sp_prop_create(sp_ctx* c, int isize, int jsize, int ksize, double dx, double
     return SP_OK;
 }
 
-int sp_prop_destroy(sp_prop* p)
+static void prop_release(sp_prop* p)
 {
-    if (!p) return SP_OK;
+    if (!p || --p->refs > 0) return;
+    (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
     for (Buf* b : {&p->scratch[0], &p->scratch[1], &p->qI, &p->k1, &p->k2, &p->k3, &p->k4, &p->nu_tab}) b->release();
+    sp_ctx* c = p->ctx;
     delete p;
+    ctx_release(c);
+}
+
+int sp_prop_destroy(sp_prop* p)
+{
+    prop_release(p);
     return SP_OK;
 }
 
@@ -753,6 +771,7 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
     SP_HIP(hipSetDevice(p->ctx->device));
     sp_parareal* pr = new sp_parareal;
     pr->p = p;
+    p->refs++;
     pr->cfg = *cfg;
     pr->S = cfg->slices;
     pr->T = cfg->end_time;
@@ -830,7 +849,9 @@ int sp_parareal_destroy(sp_parareal* pr)
     if (pr->graph) (void)hipGraphDestroy(pr->graph);
     for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->nuF, &pr->nuG, &pr->part})
         b->release();
+    sp_prop* p = pr->p;
     delete pr;
+    prop_release(p);
     return SP_OK;
 }
 
