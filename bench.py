@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: batched Parareal on one MI355X for the 128^3 / 16-slice FP64 workload.
+
+Workload (the configuration the metric is quoted on): 3-D advection-diffusion on the
+periodic unit cube, 128^3 grid, initial field = 8 Gaussians (seeded SplitMix64 draws),
+c = (1, 0.5, 0.25), nu = 5e-3, T = 1, 16 time slices batched on one GPU, fine RK4
+dt = 5e-5 (20000 steps), coarse RK4 dt = 1/1008 (1000 steps rounded up to a multiple
+of 16), K = 5 Parareal iterations, everything FP64.
+
+One "step" is one full Parareal solve (initial coarse sweep + K iterations) of that
+workload with its input already resident in device memory.  `value` is the stencil
+throughput of the solve: lattice-point updates (fine RK4 steps + coarse RK4 steps) per
+second.  The line also carries the serial-fine wall time on the same GPU, the Parareal
+speedup over it, the error of each iterate against the serial FP64 fine solution, the
+roofline of the dominant kernel (the batched fused RK4 step), and the CPU baseline (the
+oracle's serial fine sweep on the host cores, a bounded sample).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """The oracle's serial fine RK4 (reference OpenMP structure) on the host, bounded."""
+    import oracle as O
+    from stencilparareal_amd.config import rounded_steps
+    N = w.N
+    g = w.gauss
+    q0 = O.fill_gaussians(N, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    conv = O.OConvection(N, N, N, 1.0 / N, w.nu, 0.0, *w.c)
+    dt = w.T / rounded_steps(w.stepsF, w.slices)
+    t0 = O.wtime()
+    conv.rk4(q0, 0.0, dt, 2)
+    t_probe = max(O.wtime() - t0, 1e-3)
+    n = max(2, min(400, int(budget_s / (t_probe / 2))))
+    t0 = O.wtime()
+    conv.rk4(q0, 0.0, dt, n)
+    t = O.wtime() - t0
+    return {"value": round(N ** 3 * n / t / 1e9, 5), "unit": "GLUP/s", "cores": O.max_threads(),
+            "kind": "port",
+            "sample": f"serial fine RK4 (oracle, reference OpenMP structure), {N}^3, {n} steps, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-compare", action="store_true", help="skip comparator runs")
+    ap.add_argument("--graphs", type=int, default=1, help="capture each iteration as a hipGraph")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): the time slices are the batch "
+              "index of a single device; multi-GPU slice sharding is not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import stencilparareal_amd as sp
+    from stencilparareal_amd import _lib as L
+    from stencilparareal_amd.config import RuntimeConfiguration, rounded_steps, workload
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    w = workload(2)
+    N, S = w.N, w.slices
+    conf = RuntimeConfiguration(gridSize=N, endTime=w.T, timeStepsFine=w.stepsF, timeStepsCoarse=w.stepsG,
+                                nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=w.kmax, async_=False,
+                                timeSlices=S)
+    tsF, tsG = conf.timeStepsFinePerTimeSlice(), conf.timeStepsCoarsePerTimeSlice()
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
+    init, sol, serial = conv.field("qinitial"), conv.field("q"), conv.field("qreference")
+    g = w.gauss
+    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype="f64", use_graphs=bool(a.graphs))
+
+    # serial FP64 fine solution on the GPU (also the error reference)
+    log("serial fine")
+    pr.q = serial
+    pr.DoSerial()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.DoSerial()
+    torch.cuda.synchronize()
+    t_serial = time.perf_counter() - t0
+    pr.q = sol
+
+    # the timed Parareal solves
+    log(f"warmup {a.warmup}")
+    for _ in range(a.warmup):
+        pr.DoParallel()
+    torch.cuda.synchronize()
+    log(f"timed {a.steps}")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rep = pr.DoParallel()
+    torch.cuda.synchronize()
+    t_steps = time.perf_counter() - t0
+    ms = t_steps / a.steps * 1e3
+
+    lup_f = w.kmax * S * tsF * N ** 3
+    lup_g = tsG * N ** 3 * ((S - 1) + S + w.kmax * (S - 1))
+    value = (lup_f + lup_g) / (ms / 1e3) / 1e9
+
+    # error of every iterate against the serial fine solution (untimed)
+    pr.begin()
+    errs_l2, errs_inf = [sp.rel_l2(pr.solution(), serial)], [sp.error_reference(pr.solution(), serial)]
+    for _ in range(w.kmax):
+        pr.iterate()
+        errs_l2.append(sp.rel_l2(pr.solution(), serial))
+        errs_inf.append(sp.error_reference(pr.solution(), serial))
+
+    # roofline of the dominant kernel: one batched fused RK4 step over all 16 slices
+    log("roofline")
+    lay = init.layout
+    n = N ** 3
+    bin_ = torch.empty(S * n, dtype=torch.float64, device="cuda")
+    bout = torch.empty_like(bin_)
+    for b in range(S):
+        bin_[b * n:(b + 1) * n].copy_(init.interior().reshape(-1))
+    conv.ctx.sync_stream()
+    dt_f = conf.dtFine()
+    for _ in range(3):
+        L.call("sp_do_rk4_batched", conv.h, S, C_ptr(lay), ctypes_vp(bin_), C_ptr(lay), ctypes_vp(bout), n, 0.0,
+               dt_f, 1)
+    reps = 50
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for r in range(reps):
+        src, dst = (bin_, bout) if r % 2 == 0 else (bout, bin_)
+        L.call("sp_do_rk4_batched", conv.h, S, C_ptr(lay), ctypes_vp(src), C_ptr(lay), ctypes_vp(dst), n, 0.0,
+               dt_f, 1)
+    ev1.record()
+    ev1.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / reps
+    alg_bytes = 16.0 * S * n  # read + write of the FP64 state per lattice update
+    achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    del bin_, bout
+
+    extra = {}
+    if not a.no_compare:
+        log("mixed-precision coarse comparator")
+        sol2 = conv.field("q2")
+        pr32 = sp.Parareal(conv, init, sol2, conf, coarse="rk4", coarse_dtype="f32", use_graphs=bool(a.graphs))
+        pr32.DoParallel()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pr32.DoParallel()
+        torch.cuda.synchronize()
+        extra["parareal_f32_coarse_s"] = round(time.perf_counter() - t0, 4)
+        extra["parareal_f32_coarse_rel_l2_at_K"] = sp.rel_l2(sol2, serial)
+        del pr32
+
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(w)
+
+    out = {
+        "metric": "Parareal wall-time vs serial-FP64-fine at ≤1e-6 rel L2; stencil GLUP/s + %HBM peak",
+        "value": round(value, 3),
+        "unit": "GLUP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (8 seeded Gaussians, SplitMix64 seed 0)",
+        "config": {"workload": "128^3 grid, 16 time slices batched, FP64 fine RK4 dt=5e-5 + FP64 coarse RK4 "
+                               "dt=1/1008, K=5, T=1", "grid": N, "slices": S, "kmax": w.kmax,
+                   "fine_steps_per_slice": tsF, "coarse_steps_per_slice": tsG, "parallelism": "slices-as-batch"},
+        "parareal_wall_s": round(ms / 1e3, 4),
+        "serial_fine_wall_s": round(t_serial, 4),
+        "speedup_vs_serial_fine": round(t_serial / (ms / 1e3), 4),
+        "rel_l2_err_per_iterate": [float(f"{e:.4e}") for e in errs_l2],
+        "rel_linf_err_per_iterate": [float(f"{e:.4e}") for e in errs_inf],
+        "fine_glups_serial": round(N ** 3 * conf.timeStepsFine / t_serial / 1e9, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_rk4_fused<double,double,double> over 16 slices",
+                     "avg_launch_ms": round(k_ms, 4), "bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def C_ptr(lay):
+    import ctypes
+    return ctypes.byref(lay)
+
+
+def ctypes_vp(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+if __name__ == "__main__":
+    main()

@@ -127,6 +127,13 @@ int sp_do_euler_timestep(sp_prop* p, const sp_layout* lin, const void* in, const
 int sp_do_rk4_ex(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
                  double tStart, double dt, int steps);
 
+/* Batched DoRK4 over `nslices` independent fields of one layout, slice b at
+ * in + b*slice_stride (elements), all with the same start time; one fused kernel
+ * launch per step covers every slice.  in != out required (no aliasing). */
+int sp_do_rk4_batched(sp_prop* p, int nslices, const sp_layout* lin, const void* in,
+                      const sp_layout* lout, void* out, long long slice_stride, double tStart,
+                      double dt, int steps);
+
 /* ---- halo fill, update, norms -------------------------------------------------- */
 /* ApplyImpl: fills the layout's halo periodically, I faces (interior j,k), then J
  * faces (all i, interior k), then K faces (all i, j).  SP_F64, halo >= 1. */

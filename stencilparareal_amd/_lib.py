@@ -32,7 +32,8 @@ EXPORTS = [
     "sp_ctx_get_stream", "sp_ctx_sync", "sp_malloc", "sp_free", "sp_memcpy_h2d", "sp_memcpy_d2h",
     "sp_layout_compact", "sp_layout_stella_cuda", "sp_layout_storage_elems", "sp_prop_create",
     "sp_prop_destroy", "sp_prop_set_upwind", "sp_prop_set_path", "sp_prop_set_tile", "sp_do_rk4",
-    "sp_do_euler", "sp_do_rk4_timestep", "sp_do_euler_timestep", "sp_do_rk4_ex", "sp_periodic_fill",
+    "sp_do_euler", "sp_do_rk4_timestep", "sp_do_euler_timestep", "sp_do_rk4_ex", "sp_do_rk4_batched",
+    "sp_periodic_fill",
     "sp_parareal_update", "sp_norms", "sp_fill_exact", "sp_fill_gaussians", "sp_parareal_create",
     "sp_parareal_destroy", "sp_parareal_begin", "sp_parareal_iterate", "sp_parareal_solution",
     "sp_parareal_run", "sp_parareal_serial",
@@ -95,6 +96,8 @@ _SIGS = {
     "sp_do_rk4_timestep": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double]),
     "sp_do_euler_timestep": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double]),
     "sp_do_rk4_ex": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
+    "sp_do_rk4_batched": (C.c_int, [_vp, C.c_int, _lp, _vp, _lp, _vp, C.c_longlong, C.c_double, C.c_double,
+                                    C.c_int]),
     "sp_periodic_fill": (C.c_int, [_vp, _lp, _vp]),
     "sp_parareal_update": (C.c_int, [_vp, _lp, _vp, _vp, _vp, _vp]),
     "sp_norms": (C.c_int, [_vp, _lp, _vp, _lp, _vp, _dp]),

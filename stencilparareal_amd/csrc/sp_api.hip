@@ -475,6 +475,45 @@ int sp_do_rk4_timestep(sp_prop* p, const sp_layout* lin, const void* in, const s
     return rk4_common(p, lin, in, lout, out, t, dt, 1, true, "sp_do_rk4_timestep");
 }
 
+int sp_do_rk4_batched(sp_prop* p, int nslices, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
+                      long long slice_stride, double tStart, double dt, int steps)
+{
+    SP_CHECK(p, SP_ERR_INVALID, "sp_do_rk4_batched: null propagator");
+    SP_TRY(check_prop_layout(p, lin, in, "sp_do_rk4_batched"));
+    SP_TRY(check_prop_layout(p, lout, out, "sp_do_rk4_batched"));
+    SP_CHECK(nslices >= 1, SP_ERR_INVALID, "sp_do_rk4_batched: nslices < 1");
+    SP_CHECK(steps >= 0, SP_ERR_INVALID, "sp_do_rk4_batched: steps < 0");
+    SP_CHECK(lin->dtype == lout->dtype || lin->dtype == SP_F64, SP_ERR_UNSUPPORTED,
+             "sp_do_rk4_batched: dtype pair");
+    SP_CHECK(slice_stride >= sp_layout_storage_elems(lin) && slice_stride >= sp_layout_storage_elems(lout),
+             SP_ERR_INVALID, "sp_do_rk4_batched: slices overlap");
+    SP_CHECK(!same_origin(lin, in, lout, out), SP_ERR_INVALID, "sp_do_rk4_batched: in and out alias");
+    if (steps == 0) return SP_OK;
+    SP_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t s = p->ctx->stream;
+    View vin = view_of(*lin, in), vout = view_of(*lout, out);
+    vin.slice = slice_stride;
+    vout.slice = slice_stride;
+    // ping-pong between out and a batched scratch so the last step lands in out
+    const size_t bytes = (size_t)slice_stride * nslices * dtype_size(lout->dtype);
+    View A = vout;
+    if (steps > 1) {
+        SP_TRY(p->scratch[0].ensure(bytes));
+        A.p = (char*)p->scratch[0].p + lout->origin * dtype_size(lout->dtype);
+    }
+    const FusedTile tile = pick_tile(p, nslices, lin->dtype == SP_F64 && lout->dtype == SP_F64);
+    View cur = vin;
+    int cdt = lin->dtype;
+    for (int i = 0; i < steps; ++i) {
+        const View d = ((steps - 1 - i) % 2 == 0) ? vout : A;
+        SP_TRY(launch_rk4_fused(s, fused_step(p, nslices, dt, nullptr, computenu(p, tStart + i * dt)), tile, cdt, cur,
+                                lout->dtype, d));
+        cur = d;
+        cdt = lout->dtype;
+    }
+    return SP_OK;
+}
+
 int sp_do_euler(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out, double tStart,
                 double dt, int steps)
 {
