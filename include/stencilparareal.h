@@ -47,6 +47,11 @@ extern "C" {
 #define SP_PATH_FUSED 0  /* one kernel per RK4 step, 4 stages fused (default) */
 #define SP_PATH_STAGED 1 /* reference structure: 4 RHS + 3 Euler + 1 RK sweep per step */
 
+/* ---- fused-kernel variants ------------------------------------------------------ */
+#define SP_KERNEL_AUTO -1
+#define SP_KERNEL_SCALAR 0 /* one column per lane, any sizes and storage types */
+#define SP_KERNEL_PAIR 1   /* FP64, two x-adjacent columns per lane, 16-byte accesses */
+
 /*
  * Field layout.  Element (i,j,k) of the interior, 0 <= i < ni etc., lives at
  *   base[origin + i + j*sj + k*sk]
@@ -105,9 +110,12 @@ int sp_prop_destroy(sp_prop* p);
 int sp_prop_set_upwind(sp_prop* p, int upwind);
 /* SP_PATH_FUSED (default) or SP_PATH_STAGED for DoRK4. */
 int sp_prop_set_path(sp_prop* p, int path);
-/* Fused-kernel tile: tx, ty in {16, 24, 32}, nthreads in {256, 512, 1024}, kchunk
- * in 1..nk (0 for the autotuned choice). */
-int sp_prop_set_tile(sp_prop* p, int tx, int ty, int nthreads, int kchunk);
+/* Fused-kernel choice: variant SP_KERNEL_*, tile tx x ty columns, nthreads per block,
+ * kchunk planes per block (0: whole k extent).  variant SP_KERNEL_AUTO restores the
+ * built-in choice.  Compiled shapes: scalar {32x32x1024, 32x24x1024, 24x24x1024,
+ * 32x16x512, 16x16x256}, pair {32x32x1024, 32x32x768, 32x32x512, 32x24x768, 32x16x512,
+ * 16x16x256}. */
+int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int kchunk);
 
 /* DoRK4: `steps` RK4 steps of size dt from tStart; step i runs at tStart + i*dt.
  * in == out is allowed.  steps == 0 leaves out untouched.  in and out may have

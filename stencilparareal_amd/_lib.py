@@ -21,6 +21,7 @@ SP_ERR_NOMEM = 5
 SP_F64, SP_F32, SP_BF16 = 0, 1, 2
 SP_COARSE_EULER, SP_COARSE_RK4 = 0, 1
 SP_PATH_FUSED, SP_PATH_STAGED = 0, 1
+SP_KERNEL_AUTO, SP_KERNEL_SCALAR, SP_KERNEL_PAIR = -1, 0, 1
 SP_MAX_ITERS = 64
 
 DTYPE_CODES = {"f64": SP_F64, "f32": SP_F32, "bf16": SP_BF16}
@@ -90,7 +91,7 @@ _SIGS = {
     "sp_prop_destroy": (C.c_int, [_vp]),
     "sp_prop_set_upwind": (C.c_int, [_vp, C.c_int]),
     "sp_prop_set_path": (C.c_int, [_vp, C.c_int]),
-    "sp_prop_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "sp_prop_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "sp_do_rk4": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
     "sp_do_euler": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
     "sp_do_rk4_timestep": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double]),

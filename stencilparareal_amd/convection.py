@@ -34,8 +34,14 @@ class Convection:
     def set_path(self, path: str):
         L.call("sp_prop_set_path", self.h, {"fused": L.SP_PATH_FUSED, "staged": L.SP_PATH_STAGED}[path])
 
-    def set_tile(self, tx=0, ty=0, nthreads=0, kchunk=0):
-        L.call("sp_prop_set_tile", self.h, tx, ty, nthreads, kchunk)
+    def set_tile(self, tx=0, ty=0, nthreads=0, kchunk=0, variant=None):
+        """Fused-kernel tile.  variant: "scalar", "pair" or None (built-in choice when
+        tx == 0, else scalar)."""
+        if tx == 0 and variant is None:
+            v = L.SP_KERNEL_AUTO
+        else:
+            v = {"scalar": L.SP_KERNEL_SCALAR, "pair": L.SP_KERNEL_PAIR, None: L.SP_KERNEL_SCALAR}[variant]
+        L.call("sp_prop_set_tile", self.h, v, tx, ty, nthreads, kchunk)
 
     def computenu(self, t: float) -> float:
         import math

@@ -83,7 +83,7 @@ struct sp_prop {
     double dx = 0, nu0 = 0, nufreq = 0, cx = 0, cy = 0, cz = 0;
     int upwind = 1;
     int path = SP_PATH_FUSED;
-    FusedTile tile{0, 0, 0, -1};  // tx == 0: automatic
+    FusedTile tile{0, 0, 0, -1, SP_KERNEL_AUTO};  // tx == 0: automatic
     Buf scratch[2];                // ping-pong for the fused steppers
     Buf qI, k1, k2, k3, k4;         // staged path temporaries (f64, compact)
     Buf nu_tab;                     // per-step nu table
@@ -131,12 +131,13 @@ FusedTile pick_tile(const sp_prop* p, int nslices, int f64)
         t.tx = p->tile.tx;
         t.ty = p->tile.ty;
         t.nt = p->tile.nt;
+        t.variant = p->tile.variant;
     }
     if (p->tile.kchunk >= 0) t.kchunk = p->tile.kchunk;
     return t;
 }
 
-FusedStep fused_step(const sp_prop* p, int nslices, double dt, const double* nu_dev, double nu)
+FusedStep fused_step(const sp_prop* p, int nslices, double dt, const void* coef_dev, double nu)
 {
     FusedStep st;
     st.ni = p->ni;
@@ -148,7 +149,7 @@ FusedStep fused_step(const sp_prop* p, int nslices, double dt, const double* nu_
     st.cy = p->cy;
     st.cz = p->cz;
     st.dx = p->dx;
-    st.nu_dev = nu_dev;
+    st.coef_dev = coef_dev;
     st.nu = nu;
     return st;
 }
@@ -412,14 +413,16 @@ int sp_prop_set_path(sp_prop* p, int path)
     return SP_OK;
 }
 
-int sp_prop_set_tile(sp_prop* p, int tx, int ty, int nthreads, int kchunk)
+int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int kchunk)
 {
     SP_CHECK(p, SP_ERR_INVALID, "sp_prop_set_tile: null propagator");
-    if (tx == 0) {
-        p->tile = FusedTile{0, 0, 0, kchunk > 0 ? kchunk : -1};
+    if (variant == SP_KERNEL_AUTO) {
+        p->tile = FusedTile{0, 0, 0, kchunk > 0 ? kchunk : -1, SP_KERNEL_AUTO};
         return SP_OK;
     }
-    const FusedTile t{tx, ty, nthreads, kchunk};
+    SP_CHECK(variant == SP_KERNEL_SCALAR || variant == SP_KERNEL_PAIR, SP_ERR_INVALID,
+             "sp_prop_set_tile: unknown variant");
+    const FusedTile t{tx, ty, nthreads, kchunk, variant};
     SP_CHECK(fused_tile_supported(t, 1), SP_ERR_UNSUPPORTED, "sp_prop_set_tile: tile shape not compiled");
     SP_CHECK(kchunk >= 0, SP_ERR_INVALID, "sp_prop_set_tile: negative kchunk");
     p->tile = t;
@@ -638,6 +641,7 @@ int sp_fill_gaussians(sp_ctx* c, const sp_layout* l, void* base, int ng, const d
 
 // ======================================================================================
 // Batched Parareal.
+static int ensure_red(sp_ctx* c, int n);
 struct sp_parareal {
     sp_prop* p = nullptr;
     sp_parareal_cfg cfg{};
@@ -646,11 +650,12 @@ struct sp_parareal {
     double T = 0;
     int gdt = SP_F64;
     size_t n = 0;                 // points per slice
-    Buf U, F, FS, Gold, Gw0, Gw1, Q, nuF, nuG, part;
+    Buf U, F, FS, Gold, Gw0, Gw1, Q, coefF, coefG, part;
     std::vector<double> nuF_host, nuG_host;
     bool nu_const = true;
     bool begun = false;
     int iters_done = 0;
+    double u0_norm2 = 0.;  // sum of u0^2, for the blow-up check
     FusedTile tileF{}, tileG1{}, tileGS{};
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -671,7 +676,7 @@ View batch_view(void* p, const sp_parareal* pr, int dt, int slice0 = 0)
 // Coarse propagation of `nslices` batched slices, tsG sub-steps from per-slice start
 // times; step i of slice r runs at t0[r] + i*dtG (or at (first + i)*dtG from t = 0 for
 // the initial chain, first_index >= 0).  Input f64 (U) or gdt; output gdt in `dst`.
-int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View dst, View scr, const double* nu_dev_base,
+int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View dst, View scr, const void* coef_base,
                long long first_index, int steps)
 {
     sp_prop* p = pr->p;
@@ -697,7 +702,7 @@ int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View
         const int left = steps - 1 - i;
         const View d = (left % 2 == 0) ? dst : scr;
         double nu;
-        const double* nud = nullptr;
+        const void* nud = nullptr;
         if (first_index >= 0) {
             nu = computenu(p, 0. + (double)(first_index + i) * pr->dtG);
         } else if (pr->nu_const) {
@@ -706,7 +711,7 @@ int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View
             nu = pr->nuG_host[(size_t)i * pr->S + slice0];
         } else {
             nu = p->nu0;
-            nud = nu_dev_base + (size_t)i * pr->S + slice0;
+            nud = (const char*)coef_base + ((size_t)i * pr->S + slice0) * coefs_size();
         }
         if (pr->cfg.coarse_kind == SP_COARSE_RK4) {
             const FusedTile& tile = nslices == 1 ? pr->tileG1 : pr->tileGS;
@@ -749,7 +754,7 @@ int fine_sweep(sp_parareal* pr)
     for (int i = 0; i < pr->tsF; ++i) {
         const int left = pr->tsF - 1 - i;
         const View d = (left % 2 == 0) ? F : FS;
-        const double* nud = pr->nu_const ? nullptr : (const double*)pr->nuF.p + (size_t)i * pr->S;
+        const void* nud = pr->nu_const ? nullptr : (const char*)pr->coefF.p + (size_t)i * pr->S * coefs_size();
         SP_TRY(launch_rk4_fused(s, fused_step(p, pr->S, pr->dtF, nud, p->nu0), pr->tileF, SP_F64, cur, SP_F64, d));
         cur = d;
     }
@@ -771,7 +776,7 @@ int coarse_chain(sp_parareal* pr)
         } else {
             const View Ur = cview((char*)pr->U.p + (size_t)r * pr->n * 8, p->ni, p->nj);
             const View A = cview(pr->Gw0.p, p->ni, p->nj), B = cview(pr->Gw1.p, p->ni, p->nj);
-            SP_TRY(coarse_run(pr, 1, r, SP_F64, Ur, A, B, (const double*)pr->nuG.p, -1, pr->tsG));
+            SP_TRY(coarse_run(pr, 1, r, SP_F64, Ur, A, B, pr->coefG.p, -1, pr->tsG));
             cnew = A;
         }
         const View fine = cview((char*)pr->F.p + (size_t)r * pr->n * 8, p->ni, p->nj);
@@ -851,13 +856,18 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
             for (int i = 0; i < pr->tsF; ++i) pr->nuF_host[(size_t)i * pr->S + rr] = computenu(p, fine_time(pr, rr, i));
             for (int i = 0; i < pr->tsG; ++i) pr->nuG_host[(size_t)i * pr->S + rr] = computenu(p, t0 + i * pr->dtG);
         }
-        chk(pr->nuF.ensure(pr->nuF_host.size() * 8));
-        chk(pr->nuG.ensure(pr->nuG_host.size() * 8));
-        if (r == SP_OK && (hipMemcpy(pr->nuF.p, pr->nuF_host.data(), pr->nuF_host.size() * 8, hipMemcpyHostToDevice) !=
-                               hipSuccess ||
-                           hipMemcpy(pr->nuG.p, pr->nuG_host.data(), pr->nuG_host.size() * 8, hipMemcpyHostToDevice) !=
-                               hipSuccess))
-            r = fail(SP_ERR_HIP, "sp_parareal_create: nu table upload");
+        // per-step, per-slice coefficient records of the fused kernel
+        const size_t cs = coefs_size();
+        std::vector<char> cf(pr->nuF_host.size() * cs), cg(pr->nuG_host.size() * cs);
+        for (size_t i = 0; i < pr->nuF_host.size(); ++i)
+            fill_coefs(cf.data() + i * cs, pr->nuF_host[i], p->cx, p->cy, p->cz, p->dx, pr->dtF);
+        for (size_t i = 0; i < pr->nuG_host.size(); ++i)
+            fill_coefs(cg.data() + i * cs, pr->nuG_host[i], p->cx, p->cy, p->cz, p->dx, pr->dtG);
+        chk(pr->coefF.ensure(cf.size()));
+        chk(pr->coefG.ensure(cg.size()));
+        if (r == SP_OK && (hipMemcpy(pr->coefF.p, cf.data(), cf.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                           hipMemcpy(pr->coefG.p, cg.data(), cg.size(), hipMemcpyHostToDevice) != hipSuccess))
+            r = fail(SP_ERR_HIP, "sp_parareal_create: coefficient table upload");
         if (r != SP_OK) {
             sp_parareal_destroy(pr);
             return r;
@@ -886,7 +896,7 @@ int sp_parareal_destroy(sp_parareal* pr)
     (void)hipStreamSynchronize(pr->p->ctx->stream);
     if (pr->gexec) (void)hipGraphExecDestroy(pr->gexec);
     if (pr->graph) (void)hipGraphDestroy(pr->graph);
-    for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->nuF, &pr->nuG, &pr->part})
+    for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->coefF, &pr->coefG, &pr->part})
         b->release();
     sp_prop* p = pr->p;
     delete pr;
@@ -935,8 +945,21 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
                 SP_TRY(coarse_run(pr, 1, r, SP_F64, sr, dr, cview(pr->Gw0.p, p->ni, p->nj), nullptr, -1, pr->tsG));
             }
         } else {
-            SP_TRY(coarse_run(pr, pr->S, 0, SP_F64, src, dst, scr, (const double*)pr->nuG.p, -1, pr->tsG));
+            SP_TRY(coarse_run(pr, pr->S, 0, SP_F64, src, dst, scr, pr->coefG.p, -1, pr->tsG));
         }
+    }
+    // ||u0||^2: advection-diffusion never increases the L2 norm, so slice states far above
+    // it mean an unstable propagator.
+    {
+        const View u = cview(pr->U.p, p->ni, p->nj);
+        SP_TRY(ensure_red(p->ctx, kReduceBlocks * 4));
+        SP_TRY(launch_norms(s, p->ni, p->nj, p->nk, u, u, p->ctx->red, kReduceBlocks));
+        SP_HIP(hipMemcpyAsync(p->ctx->red_host, p->ctx->red, kReduceBlocks * 4 * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        SP_HIP(hipStreamSynchronize(s));
+        double n2 = 0.;
+        for (int bI = 0; bI < kReduceBlocks; ++bI) n2 += p->ctx->red_host[bI * 4 + 3];
+        pr->u0_norm2 = n2;
     }
     // The coarse prediction of the final state is the last slice's coarse value.
     SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt,
@@ -999,10 +1022,12 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
     const double inc = std::sqrt(d2) / std::sqrt(q2);
     pr->iters_done++;
     if (incr) *incr = inc;
-    // A non-finite increment, or a change a million times the size of the state itself,
-    // means an unstable propagator: report it instead of returning the data.
-    SP_CHECK(std::isfinite(inc) && inc <= 1e6, SP_ERR_DIVERGED,
-             "Parareal increment " + std::to_string(inc) + ": coarse or fine step unstable");
+    // A non-finite increment, or slice states whose L2 norm exceeds the initial data's by
+    // orders of magnitude (the exact solution's norm never grows), mean an unstable
+    // propagator: report it instead of returning the data.
+    const bool blown = !(q2 <= 1e8 * pr->S * pr->u0_norm2);
+    SP_CHECK(std::isfinite(inc) && !blown, SP_ERR_DIVERGED,
+             "Parareal diverged (increment " + std::to_string(inc) + "): coarse or fine step unstable");
     return SP_OK;
 }
 

@@ -21,63 +21,88 @@
 //    so only the ring needed by later stages is recomputed.  Each stage maps its own
 //    region flat onto the block's threads, so inactive lanes are only the tail.
 //  * The state q needed by each stage's update (v_s = q + ...) is re-read from global
-//    memory (an L2 hit: the block loaded it a few planes earlier), keeping the per-column
-//    register state at 4 partial sums per stage.
+//    memory one plane ahead of its use (an L2 hit: the block loaded it a few planes
+//    earlier), keeping the per-column register state at 4 partial sums per stage.
+//  * The four running sums of a column are a circular buffer indexed by plane mod 4 and
+//    the k loop is unrolled by 4, so advancing a plane moves no registers.
+//  * The 13 stencil coefficients and the stage factors are computed on the host and
+//    arrive as kernel arguments (scalar registers); global accesses are buffer
+//    instructions with the plane offset in a scalar register.
 //  * HBM traffic per step: one read and one write of the state (2*sizeof(storage) bytes
 //    per lattice update) plus the x-y halo overlap of neighbouring tiles.
 //  * The plane for iteration t+1 is loaded into registers while iteration t computes.
 
 #include "sp_internal.hpp"
 
+#ifndef SP_SERIAL_SLOTS
+#define SP_SERIAL_SLOTS 1
+#endif
+
 namespace sp {
 
 namespace {
 
+// Host-computed coefficients of one step: the 13 star weights of L (offsets -2,-1,+1,+2
+// per axis, then the centre) and the stage factors h/4, h/3, h/2, h, in both precisions.
+struct Coefs {
+    double wd[13];
+    double fd[4];
+    float wf[13];
+    float ff[4];
+};
+
 struct FusedArgs {
     View in, out;
+    unsigned in_bytes, out_bytes;  // bytes one slice's storage spans from its origin
     int ni, nj, nk;
-    int kchunk;
-    int tiles_x;
-    double h;
-    double cx, cy, cz, dx;
-    const double* nu_dev;
-    double nu;
+    int kchunk, chunks;
+    int tiles_x, tiles;
+    Coefs c;                 // used when cdev == nullptr
+    const Coefs* cdev;       // per-slice coefficients (time-dependent nu), or nullptr
 };
 
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// The 13 star coefficients of L, unscaled: every stage shares them (uniform values that
-// live in scalar registers); the stage factor h/4, h/3, h/2, h enters in the update.
 template <typename T>
 struct StarW {
     T xm2, xm1, xp1, xp2, ym2, ym1, yp1, yp2, zm2, zm1, zp1, zp2, c;
+    T f1, f2, f3, f4;
 };
 
 template <typename T>
-__device__ __forceinline__ StarW<T> star_w(const Coef13& k)
+__device__ __forceinline__ StarW<T> load_w(const Coefs& k);
+template <>
+__device__ __forceinline__ StarW<double> load_w<double>(const Coefs& k)
 {
-    StarW<T> w;
-    w.xm2 = (T)k.x[0];
-    w.xm1 = (T)k.x[1];
-    w.xp1 = (T)k.x[2];
-    w.xp2 = (T)k.x[3];
-    w.ym2 = (T)k.y[0];
-    w.ym1 = (T)k.y[1];
-    w.yp1 = (T)k.y[2];
-    w.yp2 = (T)k.y[3];
-    w.zm2 = (T)k.z[0];
-    w.zm1 = (T)k.z[1];
-    w.zp1 = (T)k.z[2];
-    w.zp2 = (T)k.z[3];
-    w.c = (T)k.c;
-    return w;
+    return StarW<double>{k.wd[0], k.wd[1], k.wd[2], k.wd[3], k.wd[4], k.wd[5], k.wd[6], k.wd[7], k.wd[8],
+                         k.wd[9], k.wd[10], k.wd[11], k.wd[12], k.fd[0], k.fd[1], k.fd[2], k.fd[3]};
+}
+template <>
+__device__ __forceinline__ StarW<float> load_w<float>(const Coefs& k)
+{
+    return StarW<float>{k.wf[0], k.wf[1], k.wf[2], k.wf[3], k.wf[4], k.wf[5], k.wf[6], k.wf[7], k.wf[8],
+                        k.wf[9], k.wf[10], k.wf[11], k.wf[12], k.ff[0], k.ff[1], k.ff[2], k.ff[3]};
 }
 
 __device__ __forceinline__ int pmod(int a, int n)
 {
     int r = a % n;
     return r < 0 ? r + n : r;
+}
+
+// Plane index p in [-3n, 4n) wrapped into [0, n) without branches (scalar selects).  The
+// stream touches planes z0-18 .. zend+16, inside that range for n >= 8; the host sends
+// smaller k extents to the reference-order path.
+__device__ __forceinline__ int wrapz(int p, int n)
+{
+    p += p < 0 ? n : 0;
+    p += p < 0 ? n : 0;
+    p += p < 0 ? n : 0;
+    p -= p >= n ? n : 0;
+    p -= p >= n ? n : 0;
+    p -= p >= n ? n : 0;
+    return p;
 }
 
 // Stage geometry: stage s (1..4) computes a (TX+2h) x (TY+2h) region, h = 8-2s, from an
@@ -89,35 +114,128 @@ struct Geo {
     static constexpr int WI = WO + 4, HI = HOo + 4, AI = WI * HI;
 };
 
-// One stage at one plane: consume input plane p (in LDS), emit plane p-2.
-template <typename T, int TX, int TY, int S, int NT, int M>
-__device__ __forceinline__ void stage_step(const T* __restrict__ P, const StarW<T>& w, T (&acc)[M][4],
-                                           const int (&cidx)[M], T (&done)[M])
+// One stage at one plane: consume input plane p (in LDS), emit the completed sum of
+// plane p-2 in done[].  The running sums of points p-2 .. p+1 sit in slots (x mod 4)
+// of acc; R is the slot of point p.  Every slot m is processed (slots past the region's
+// end repeat its last column), so the stream has no per-lane branches.
+template <typename T, int TX, int TY, int S, int NT, int M, int R>
+__device__ __forceinline__ void stage_step(const T* __restrict__ P, const StarW<T>& w, T (&acc)[M][4], T (&done)[M])
 {
     using G = Geo<TX, TY, S>;
     constexpr int WI = G::WI;
+    constexpr int sm2 = (R + 2) & 3, sm1 = (R + 3) & 3, s0 = R & 3, sp1 = (R + 1) & 3;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const int f = threadIdx.x + m * NT;
-        if (M * NT > G::AO && f >= G::AO) break;
-        const int ci = cidx[m];
-        const T u = P[ci];
-        T a1 = acc[m][2];
-        a1 = fmaT(w.c, u, a1);
-        a1 = fmaT(w.xm2, P[ci - 2], a1);
-        a1 = fmaT(w.xm1, P[ci - 1], a1);
-        a1 = fmaT(w.xp1, P[ci + 1], a1);
-        a1 = fmaT(w.xp2, P[ci + 2], a1);
-        a1 = fmaT(w.ym2, P[ci - 2 * WI], a1);
-        a1 = fmaT(w.ym1, P[ci - WI], a1);
-        a1 = fmaT(w.yp1, P[ci + WI], a1);
-        a1 = fmaT(w.yp2, P[ci + 2 * WI], a1);
-        done[m] = fmaT(w.zp2, u, acc[m][0]);      // point p-2: its +2 neighbour is p
-        acc[m][0] = fmaT(w.zp1, u, acc[m][1]);    // point p-1
-        acc[m][1] = a1;                          // point p
-        acc[m][2] = fmaT(w.zm1, u, acc[m][3]);    // point p+1
-        acc[m][3] = w.zm2 * u;                   // point p+2
+        // LDS centre of this slot's column (recomputed: constant divisors, saves registers)
+        const int f = min((int)threadIdx.x + m * NT, G::AO - 1);
+        const T* c = P + (f / G::WO + 2) * WI + f % G::WO + 2;
+        const T u = c[0];
+        const T xm2 = c[-2], xm1 = c[-1], xp1 = c[1], xp2 = c[2];
+        const T ym2 = c[-2 * WI], ym1 = c[-WI], yp1 = c[WI], yp2 = c[2 * WI];
+        done[m] = fmaT(w.zp2, u, acc[m][sm2]);            // point p-2: its +2 neighbour is p
+        acc[m][sm2] = w.zm2 * u;                          // point p+2 starts in the same slot
+        acc[m][sm1] = fmaT(w.zp1, u, acc[m][sm1]);        // point p-1
+        // point p: centre and x-y neighbours, as two independent chains (x, y)
+        T ax = fmaT(w.c, u, acc[m][s0]);
+        T ay = w.ym2 * ym2;
+        ax = fmaT(w.xm2, xm2, ax);
+        ay = fmaT(w.ym1, ym1, ay);
+        ax = fmaT(w.xm1, xm1, ax);
+        ay = fmaT(w.yp1, yp1, ay);
+        ax = fmaT(w.xp1, xp1, ax);
+        ay = fmaT(w.yp2, yp2, ay);
+        ax = fmaT(w.xp2, xp2, ax);
+        acc[m][s0] = ax + ay;
+        acc[m][sp1] = fmaT(w.zm1, u, acc[m][sp1]);        // point p+1
+        // one slot's neighbour values in flight at a time keeps the register footprint
+        // small enough for 16 waves per CU; the other waves hide the LDS latency
+        if (SP_SERIAL_SLOTS) __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// Buffer-resource access: one SGPR descriptor per field, 32-bit per-lane byte offsets
+// and the plane offset in an SGPR, so the k stream needs no 64-bit address arithmetic.
+// Out-of-range offsets read 0 / drop the store instead of faulting.
+#define SP_RSRC_DWORD3 0x00020000  // gfx9 family raw buffer (as composable_kernel uses)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, SP_RSRC_DWORD3);
+}
+
+template <typename S>
+struct Buf;
+template <>
+struct Buf<double> {
+    using T = double;
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __device__ __forceinline__ static double ld(__amdgpu_buffer_rsrc_t r, int vo, int so)
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    }
+    __device__ __forceinline__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, double x)
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, vo, so, 0);
+    }
+};
+template <>
+struct Buf<float> {
+    __device__ __forceinline__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    }
+    __device__ __forceinline__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float x)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, so, 0);
+    }
+};
+template <>
+struct Buf<bf16> {
+    __device__ __forceinline__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so)
+    {
+        return bf16_to_f32((uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+    }
+    __device__ __forceinline__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float x)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16((unsigned short)f32_to_bf16(x), r, vo, so, 0);
+    }
+};
+
+// Per-column constants of one stage: LDS centre index and the byte offset of the column
+// in the input field (for the q value of the update).  Slots past the region's end repeat
+// its last column.
+template <int M>
+struct Cols {
+    int q[M];
+};
+
+template <typename SI, int TX, int TY, int S, int NT, int M>
+__device__ __forceinline__ void stage_cols(Cols<M>& cl, int x0, int y0, int ni, int nj, int sj)
+{
+    using G = Geo<TX, TY, S>;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int f = min((int)threadIdx.x + m * NT, G::AO - 1);
+        const int ax = f % G::WO, by = f / G::WO;
+        cl.q[m] = (pmod(x0 - G::HO + ax, ni) + pmod(y0 - G::HO + by, nj) * sj) * (int)sizeof(SI);
+    }
+}
+
+// Loads the q values a stage needs for its update at the plane whose byte offset is zo.
+template <typename SI, typename T, int M>
+__device__ __forceinline__ void stage_qload(T (&qv)[M], __amdgpu_buffer_rsrc_t r, int zo, const Cols<M>& cl)
+{
+#pragma unroll
+    for (int m = 0; m < M; ++m) qv[m] = (T)Buf<SI>::ld(r, cl.q[m], zo);
+}
+
+// Blocks are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).  Renumber so
+// that each XCD receives a contiguous range of tiles: neighbouring tiles, which read each
+// other's halo columns, then share one L2.
+__device__ __forceinline__ int xcd_block(int b, int n)
+{
+    if (n % 8 != 0) return b;
+    return (b % 8) * (n / 8) + b / 8;
 }
 
 template <typename SI, typename SO, typename T, int TX, int TY, int NT>
@@ -134,26 +252,28 @@ __global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
     constexpr int M3 = (G3::AO + NT - 1) / NT;
     constexpr int M4 = (G4::AO + NT - 1) / NT;
 
-    __shared__ T P0[G1::AI];
-    __shared__ T P1[G2::AI];
-    __shared__ T P2[G3::AI];
-    __shared__ T P3[G4::AI];
+    // Every plane buffer is doubled: in iteration t stage s reads what stage s-1 wrote in
+    // iteration t-1, so all four stages run between one barrier per plane.
+    __shared__ T P0[2][G1::AI];
+    __shared__ T P1[2][G2::AI];
+    __shared__ T P2[2][G3::AI];
+    __shared__ T P3[2][G4::AI];
 
     const int tid = threadIdx.x;
     const int ni = a.ni, nj = a.nj, nk = a.nk;
-    const int tile = blockIdx.x;
+    // block -> (tile, k chunk, slice); tiles fastest
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int tile = bid % a.tiles;
+    const int chunk = (bid / a.tiles) % a.chunks;
+    const int slice = bid / (a.tiles * a.chunks);
     const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
-    const int z0 = blockIdx.y * a.kchunk;
+    const int z0 = chunk * a.kchunk;
     const int zend = min(z0 + a.kchunk, nk);
-    const int slice = blockIdx.z;
-    const SI* __restrict__ qin = (const SI*)a.in.p + slice * a.in.slice;
-    SO* __restrict__ qout = (SO*)a.out.p + slice * a.out.slice;
-    const long long isj = a.in.sj, isk = a.in.sk, osj = a.out.sj, osk = a.out.sk;
-
-    const double nu = a.nu_dev ? a.nu_dev[slice] : a.nu;
-    const Coef13 K = make_coef13(nu, a.cx, a.cy, a.cz, a.dx);
-    const StarW<T> w = star_w<T>(K);
-    const T f1 = (T)(0.25 * a.h), f2 = (T)(a.h / 3.), f3 = (T)(0.5 * a.h), f4 = (T)a.h;
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc((const SI*)a.in.p + slice * a.in.slice, a.in_bytes);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc((SO*)a.out.p + slice * a.out.slice, a.out_bytes);
+    const int iskb = (int)a.in.sk * (int)sizeof(SI), oskb = (int)a.out.sk * (int)sizeof(SO);
+    const int isj = (int)a.in.sj, osj = (int)a.out.sj;
+    const StarW<T> w = a.cdev ? load_w<T>(a.cdev[slice]) : load_w<T>(a.c);
 
     // Per-column offsets, fixed for the whole k stream.
     int off0[M0];
@@ -161,45 +281,29 @@ __global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
     for (int m = 0; m < M0; ++m) {
         const int f = min(tid + m * NT, A0 - 1);
         const int ax = f % G1::WI, by = f / G1::WI;
-        off0[m] = pmod(x0 - 8 + ax, ni) + pmod(y0 - 8 + by, nj) * (int)isj;
+        off0[m] = (pmod(x0 - 8 + ax, ni) + pmod(y0 - 8 + by, nj) * isj) * (int)sizeof(SI);
     }
-    int c1[M1], c2[M2], c3[M3], c4[M4];
-    int q1[M1], q2[M2], q3[M3], q4[M4], o4[M4];
-    bool w4ok[M4];
-#pragma unroll
-    for (int m = 0; m < M1; ++m) {
-        const int f = min(tid + m * NT, G1::AO - 1);
-        const int ax = f % G1::WO, by = f / G1::WO;
-        c1[m] = (by + 2) * G1::WI + ax + 2;
-        q1[m] = pmod(x0 - G1::HO + ax, ni) + pmod(y0 - G1::HO + by, nj) * (int)isj;
-    }
-#pragma unroll
-    for (int m = 0; m < M2; ++m) {
-        const int f = min(tid + m * NT, G2::AO - 1);
-        const int ax = f % G2::WO, by = f / G2::WO;
-        c2[m] = (by + 2) * G2::WI + ax + 2;
-        q2[m] = pmod(x0 - G2::HO + ax, ni) + pmod(y0 - G2::HO + by, nj) * (int)isj;
-    }
-#pragma unroll
-    for (int m = 0; m < M3; ++m) {
-        const int f = min(tid + m * NT, G3::AO - 1);
-        const int ax = f % G3::WO, by = f / G3::WO;
-        c3[m] = (by + 2) * G3::WI + ax + 2;
-        q3[m] = pmod(x0 - G3::HO + ax, ni) + pmod(y0 - G3::HO + by, nj) * (int)isj;
-    }
+    Cols<M1> c1;
+    Cols<M2> c2;
+    Cols<M3> c3;
+    Cols<M4> c4;
+    stage_cols<SI, TX, TY, 1, NT, M1>(c1, x0, y0, ni, nj, isj);
+    stage_cols<SI, TX, TY, 2, NT, M2>(c2, x0, y0, ni, nj, isj);
+    stage_cols<SI, TX, TY, 3, NT, M3>(c3, x0, y0, ni, nj, isj);
+    stage_cols<SI, TX, TY, 4, NT, M4>(c4, x0, y0, ni, nj, isj);
+    int o4[M4];
 #pragma unroll
     for (int m = 0; m < M4; ++m) {
-        const int f = min(tid + m * NT, G4::AO - 1);
-        const int ax = f % G4::WO, by = f / G4::WO;
-        c4[m] = (by + 2) * G4::WI + ax + 2;
-        const int gx = x0 + ax, gy = y0 + by;
-        q4[m] = pmod(gx, ni) + pmod(gy, nj) * (int)isj;
-        o4[m] = gx + gy * (int)osj;
-        w4ok[m] = (tid + m * NT < G4::AO) && gx < ni && gy < nj;
+        const int fc = min(tid + m * NT, G4::AO - 1);
+        const int gx = x0 + fc % G4::WO, gy = y0 + fc / G4::WO;
+        // columns of a partial tile past the domain edge are written to their periodic image
+        // (where another block writes the same bits)
+        o4[m] = (pmod(gx, ni) + pmod(gy, nj) * osj) * (int)sizeof(SO);
     }
 
     T acc1[M1][4], acc2[M2][4], acc3[M3][4], acc4[M4][4];
     T d1[M1], d2[M2], d3[M3], d4[M4];
+    T qv1[M1], qv2[M2], qv3[M3], qv4[M4];  // q for the update, loaded one plane ahead
 #pragma unroll
     for (int m = 0; m < M1; ++m) acc1[m][0] = acc1[m][1] = acc1[m][2] = acc1[m][3] = T(0);
 #pragma unroll
@@ -209,102 +313,416 @@ __global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
 #pragma unroll
     for (int m = 0; m < M4; ++m) acc4[m][0] = acc4[m][1] = acc4[m][2] = acc4[m][3] = T(0);
 
-    const int tbeg = z0 - 8, tend = zend + 8;
-    T pre[M0];
+    // Iteration t: stage 1 takes q plane t and emits v1 plane t-2; stage 2 takes v1 plane
+    // t-3 (written in iteration t-1) and emits v2 plane t-5; stage 3: v2 t-6 -> v3 t-8;
+    // stage 4: v3 t-9 -> q' t-11.  Stage s runs while its input planes are valid.
+    const int tq0 = z0 - 8, tq1 = zend + 8;  // q planes to stream
+    const int tend = zend + 11;
     {
-        const long long zo = pmod(tbeg, nk) * isk;
+        // q plane tq0 into P0[tq0 & 1]
+        const int zo = wrapz(tq0, nk) * iskb;
 #pragma unroll
-        for (int m = 0; m < M0; ++m) pre[m] = (T)Store<SI>::load(qin + zo + off0[m]);
+        for (int m = 0; m < M0; ++m) P0[tq0 & 1][min(tid + m * NT, A0 - 1)] = (T)Buf<SI>::ld(rin, off0[m], zo);
+        stage_qload<SI, T, M1>(qv1, rin, wrapz(tq0 - 2, nk) * iskb, c1);
+        stage_qload<SI, T, M2>(qv2, rin, wrapz(z0 - 8, nk) * iskb, c2);
+        stage_qload<SI, T, M3>(qv3, rin, wrapz(z0 - 6, nk) * iskb, c3);
+        stage_qload<SI, T, M4>(qv4, rin, wrapz(z0 - 4, nk) * iskb, c4);
     }
+    T pre[M0];
+    if (tq0 + 1 < tq1) {
+        const int zo = wrapz(tq0 + 1, nk) * iskb;
+#pragma unroll
+        for (int m = 0; m < M0; ++m) pre[m] = (T)Buf<SI>::ld(rin, off0[m], zo);
+    }
+    __syncthreads();
 
-    for (int t = tbeg; t < tend; ++t) {
-        // Plane t of q into LDS; start loading plane t+1.
+    // One plane of the stream; R = (t - tq0) mod 4 is a compile-time constant so the slots of
+    // the running sums are static register names.  Slot of point x = (x - tq0) mod 4.
+    auto plane = [&](auto Rc, int t) {
+        constexpr int R = decltype(Rc)::value;
+        const int cur = t & 1, prv = cur ^ 1;
+        __builtin_amdgcn_sched_barrier(0);  // keep planes apart: bounded live ranges
+        // next q plane from registers into the other P0 buffer; start loading the one after
+        if (t + 1 < tq1) {
 #pragma unroll
-        for (int m = 0; m < M0; ++m) {
-            const int f = tid + m * NT;
-            if (M0 * NT == A0 || f < A0) P0[f] = pre[m];
-        }
-        if (t + 1 < tend) {
-            const long long zo = pmod(t + 1, nk) * isk;
+            for (int m = 0; m < M0; ++m) P0[prv][min(tid + m * NT, A0 - 1)] = pre[m];
+            if (t + 2 < tq1) {
+                const int zo = wrapz(t + 2, nk) * iskb;
 #pragma unroll
-            for (int m = 0; m < M0; ++m) pre[m] = (T)Store<SI>::load(qin + zo + off0[m]);
-        }
-        __syncthreads();
-
-        // Stage 1: input q plane t, emits v1 plane t-2 into P1.
-        {
-            stage_step<T, TX, TY, 1, NT, M1>(P0, w, acc1, c1, d1);
-            const long long zo = pmod(t - 2, nk) * isk;
-#pragma unroll
-            for (int m = 0; m < M1; ++m) {
-                const int f = tid + m * NT;
-                if (M1 * NT == G1::AO || f < G1::AO)
-                    P1[f] = fmaT(f1, d1[m], (T)Store<SI>::load(qin + zo + q1[m]));
+                for (int m = 0; m < M0; ++m) pre[m] = (T)Buf<SI>::ld(rin, off0[m], zo);
             }
         }
-        __syncthreads();
-
-        // Stage 2: input v1 plane t-2, emits v2 plane t-4.
-        if (t >= tbeg + 4) {
-            stage_step<T, TX, TY, 2, NT, M2>(P1, w, acc2, c2, d2);
-            const long long zo = pmod(t - 4, nk) * isk;
+        __builtin_amdgcn_sched_barrier(0);
+        // Stage 1: q plane t -> v1 plane t-2.
+        if (t < tq1) {
+            stage_step<T, TX, TY, 1, NT, M1, R & 3>(P0[cur], w, acc1, d1);
 #pragma unroll
-            for (int m = 0; m < M2; ++m) {
-                const int f = tid + m * NT;
-                if (M2 * NT == G2::AO || f < G2::AO)
-                    P2[f] = fmaT(f2, d2[m], (T)Store<SI>::load(qin + zo + q2[m]));
+            for (int m = 0; m < M1; ++m) P1[cur][min(tid + m * NT, G1::AO - 1)] = fmaT(w.f1, d1[m], qv1[m]);
+            stage_qload<SI, T, M1>(qv1, rin, wrapz(t - 1, nk) * iskb, c1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // Stage 2: v1 plane t-3 -> v2 plane t-5.
+        if (t >= tq0 + 5 && t < tq1 + 1) {
+            stage_step<T, TX, TY, 2, NT, M2, (R + 1) & 3>(P1[prv], w, acc2, d2);
+#pragma unroll
+            for (int m = 0; m < M2; ++m) P2[cur][min(tid + m * NT, G2::AO - 1)] = fmaT(w.f2, d2[m], qv2[m]);
+            stage_qload<SI, T, M2>(qv2, rin, wrapz(t - 4, nk) * iskb, c2);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // Stage 3: v2 plane t-6 -> v3 plane t-8.
+        if (t >= tq0 + 10 && t < tq1 + 2) {
+            stage_step<T, TX, TY, 3, NT, M3, (R + 2) & 3>(P2[prv], w, acc3, d3);
+#pragma unroll
+            for (int m = 0; m < M3; ++m) P3[cur][min(tid + m * NT, G3::AO - 1)] = fmaT(w.f3, d3[m], qv3[m]);
+            stage_qload<SI, T, M3>(qv3, rin, wrapz(t - 7, nk) * iskb, c3);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // Stage 4: v3 plane t-9 -> q' plane t-11.
+        if (t >= tq0 + 15) {
+            stage_step<T, TX, TY, 4, NT, M4, (R + 3) & 3>(P3[prv], w, acc4, d4);
+            const int zp = t - 11;
+            if (zp >= z0 && zp < zend) {
+                const int zo = zp * oskb;
+#pragma unroll
+                for (int m = 0; m < M4; ++m) Buf<SO>::st(rout, o4[m], zo, fmaT(w.f4, d4[m], qv4[m]));
             }
+            stage_qload<SI, T, M4>(qv4, rin, wrapz(t - 10, nk) * iskb, c4);
+            __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+    };
 
-        // Stage 3: input v2 plane t-4, emits v3 plane t-6.
-        if (t >= tbeg + 8) {
-            stage_step<T, TX, TY, 3, NT, M3>(P2, w, acc3, c3, d3);
-            const long long zo = pmod(t - 6, nk) * isk;
-#pragma unroll
-            for (int m = 0; m < M3; ++m) {
-                const int f = tid + m * NT;
-                if (M3 * NT == G3::AO || f < G3::AO)
-                    P3[f] = fmaT(f3, d3[m], (T)Store<SI>::load(qin + zo + q3[m]));
-            }
-        }
-        __syncthreads();
-
-        // Stage 4: input v3 plane t-6, emits q' plane t-8.
-        if (t >= tbeg + 12) {
-            stage_step<T, TX, TY, 4, NT, M4>(P3, w, acc4, c4, d4);
-            const int zp = t - 8;
-            if (zp >= z0) {
-                const long long zi = pmod(zp, nk) * isk;
-                const long long zo = (long long)zp * osk;
-#pragma unroll
-                for (int m = 0; m < M4; ++m)
-                    if (w4ok[m]) Store<SO>::store(qout + zo + o4[m], fmaT(f4, d4[m], (T)Store<SI>::load(qin + zi + q4[m])));
-            }
-        }
+    // Stage s runs while its input planes are valid.  The loop is unrolled by 4 so that
+    // the phase R of every plane is a compile-time constant.
+    for (int t = tq0; t < tend; t += 4) {
+        plane(std::integral_constant<int, 0>{}, t);
+        if (t + 1 < tend) plane(std::integral_constant<int, 1>{}, t + 1);
+        if (t + 2 < tend) plane(std::integral_constant<int, 2>{}, t + 2);
+        if (t + 3 < tend) plane(std::integral_constant<int, 3>{}, t + 3);
     }
 }
 
-template <typename SI, typename SO, typename T>
-int launch_typed(hipStream_t s, const FusedStep& st, const FusedTile& tile, View in, View out)
+
+// ---- FP64 x-pair variant ----------------------------------------------------------------
+// Each lane owns two x-adjacent columns, so every LDS and global access moves 16 bytes
+// (ds_read_b128 / buffer_load_dwordx4): per pair and stage, 7 LDS reads give the two centres,
+// the four x and eight y neighbours.  Needs even ni and even, 16-byte aligned strides.
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ d2 ld2(__amdgpu_buffer_rsrc_t r, int vo, int so)
 {
-    FusedArgs a;
+    return __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0));
+}
+__device__ __forceinline__ void st2(__amdgpu_buffer_rsrc_t r, int vo, int so, d2 x)
+{
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, vo, so, 0);
+}
+
+// Pair g of a stage region -> element offset of its first column in the region (row-major,
+// WO columns per row, WO even): simply 2g.  Its LDS centre in the input plane is
+// (by + 2) * WI + ax + 2 with ax = 2 (g % (WO/2)), by = g / (WO/2).
+template <int TX, int TY, int S>
+__device__ __forceinline__ int pair_centre(int g)
+{
+    using G = Geo<TX, TY, S>;
+    constexpr int WP = G::WO / 2;
+    return (g / WP + 2) * G::WI + 2 * (g % WP) + 2;
+}
+
+template <int TX, int TY, int S, int NT, int M, int R>
+__device__ __forceinline__ void pair_step(const double* __restrict__ P, const StarW<double>& w, double (&acc)[M][2][4],
+                                          double (&done)[M][2])
+{
+    using G = Geo<TX, TY, S>;
+    constexpr int WI = G::WI, AP = G::AO / 2;
+    constexpr int sm2 = (R + 2) & 3, sm1 = (R + 3) & 3, s0 = R & 3, sp1 = (R + 1) & 3;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int g = min((int)threadIdx.x + m * NT, AP - 1);
+        const double* c = P + pair_centre<TX, TY, S>(g);
+        // x group first (centres and x neighbours), then the y group: at most 4 pairs of
+        // loaded values are live at a time
+        const d2 C = *(const d2*)c, Lx = *(const d2*)(c - 2), Rx = *(const d2*)(c + 2);
+        double ax[2];
+        {
+            const double u[2] = {C.x, C.y};
+            // x neighbours of the two columns: (Lx.x, Lx.y, C.y, Rx.x) and (Lx.y, C.x, Rx.x, Rx.y)
+            const double xm2[2] = {Lx.x, Lx.y}, xm1[2] = {Lx.y, C.x}, xp1[2] = {C.y, Rx.x}, xp2[2] = {Rx.x, Rx.y};
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                done[m][e] = fmaT(w.zp2, u[e], acc[m][e][sm2]);
+                acc[m][e][sm2] = w.zm2 * u[e];
+                acc[m][e][sm1] = fmaT(w.zp1, u[e], acc[m][e][sm1]);
+                acc[m][e][sp1] = fmaT(w.zm1, u[e], acc[m][e][sp1]);
+                double a = fmaT(w.c, u[e], acc[m][e][s0]);
+                a = fmaT(w.xm2, xm2[e], a);
+                a = fmaT(w.xm1, xm1[e], a);
+                a = fmaT(w.xp1, xp1[e], a);
+                ax[e] = fmaT(w.xp2, xp2[e], a);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            const d2 Ym2 = *(const d2*)(c - 2 * WI), Ym1 = *(const d2*)(c - WI);
+            const d2 Yp1 = *(const d2*)(c + WI), Yp2 = *(const d2*)(c + 2 * WI);
+            const double ym2[2] = {Ym2.x, Ym2.y}, ym1[2] = {Ym1.x, Ym1.y}, yp1[2] = {Yp1.x, Yp1.y},
+                         yp2[2] = {Yp2.x, Yp2.y};
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                double a = w.ym2 * ym2[e];
+                a = fmaT(w.ym1, ym1[e], a);
+                a = fmaT(w.yp1, yp1[e], a);
+                a = fmaT(w.yp2, yp2[e], a);
+                acc[m][e][s0] = ax[e] + a;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int TX, int TY, int S, int NT, int M>
+__device__ __forceinline__ void pair_offsets(int (&q)[M], int x0, int y0, int ni, int nj, int sj)
+{
+    using G = Geo<TX, TY, S>;
+    constexpr int WP = G::WO / 2, AP = G::AO / 2;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int g = min((int)threadIdx.x + m * NT, AP - 1);
+        const int ax = 2 * (g % WP), by = g / WP;
+        q[m] = (pmod(x0 - G::HO + ax, ni) + pmod(y0 - G::HO + by, nj) * sj) * 8;
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void pair_qload(d2 (&qv)[M], __amdgpu_buffer_rsrc_t r, int zo, const int (&q)[M])
+{
+#pragma unroll
+    for (int m = 0; m < M; ++m) qv[m] = ld2(r, q[m], zo);
+}
+
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
+{
+    using G1 = Geo<TX, TY, 1>;
+    using G2 = Geo<TX, TY, 2>;
+    using G3 = Geo<TX, TY, 3>;
+    using G4 = Geo<TX, TY, 4>;
+    constexpr int A0P = G1::AI / 2;  // pairs of the q plane incl. its 8-cell halo
+    constexpr int M0 = (A0P + NT - 1) / NT;
+    constexpr int M1 = (G1::AO / 2 + NT - 1) / NT;
+    constexpr int M2 = (G2::AO / 2 + NT - 1) / NT;
+    constexpr int M3 = (G3::AO / 2 + NT - 1) / NT;
+    constexpr int M4 = (G4::AO / 2 + NT - 1) / NT;
+
+    __shared__ __attribute__((aligned(16))) double P0[2][G1::AI];
+    __shared__ __attribute__((aligned(16))) double P1[2][G2::AI];
+    __shared__ __attribute__((aligned(16))) double P2[2][G3::AI];
+    __shared__ __attribute__((aligned(16))) double P3[2][G4::AI];
+
+    const int tid = threadIdx.x;
+    const int ni = a.ni, nj = a.nj, nk = a.nk;
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int tile = bid % a.tiles;
+    const int chunk = (bid / a.tiles) % a.chunks;
+    const int slice = bid / (a.tiles * a.chunks);
+    const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
+    const int z0 = chunk * a.kchunk;
+    const int zend = min(z0 + a.kchunk, nk);
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc((const double*)a.in.p + slice * a.in.slice, a.in_bytes);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc((double*)a.out.p + slice * a.out.slice, a.out_bytes);
+    const int iskb = (int)a.in.sk * 8, oskb = (int)a.out.sk * 8;
+    const int isj = (int)a.in.sj, osj = (int)a.out.sj;
+    const StarW<double> w = a.cdev ? load_w<double>(a.cdev[slice]) : load_w<double>(a.c);
+
+    int off0[M0];
+#pragma unroll
+    for (int m = 0; m < M0; ++m) {
+        const int g = min(tid + m * NT, A0P - 1);
+        const int ax = 2 * (g % (G1::WI / 2)), by = g / (G1::WI / 2);
+        off0[m] = (pmod(x0 - 8 + ax, ni) + pmod(y0 - 8 + by, nj) * isj) * 8;
+    }
+    int q1[M1], q2[M2], q3[M3], q4[M4], o4[M4];
+    pair_offsets<TX, TY, 1, NT, M1>(q1, x0, y0, ni, nj, isj);
+    pair_offsets<TX, TY, 2, NT, M2>(q2, x0, y0, ni, nj, isj);
+    pair_offsets<TX, TY, 3, NT, M3>(q3, x0, y0, ni, nj, isj);
+    pair_offsets<TX, TY, 4, NT, M4>(q4, x0, y0, ni, nj, isj);
+#pragma unroll
+    for (int m = 0; m < M4; ++m) {
+        const int g = min(tid + m * NT, G4::AO / 2 - 1);
+        const int gx = x0 + 2 * (g % (TX / 2)), gy = y0 + g / (TX / 2);
+        o4[m] = (pmod(gx, ni) + pmod(gy, nj) * osj) * 8;  // periodic image for partial tiles
+    }
+
+    double acc1[M1][2][4], acc2[M2][2][4], acc3[M3][2][4], acc4[M4][2][4];
+    double d1[M1][2], d2v[M2][2], d3[M3][2], d4[M4][2];
+    d2 qv1[M1], qv2[M2], qv3[M3], qv4[M4];
+#pragma unroll
+    for (int m = 0; m < M1; ++m)
+        for (int e = 0; e < 2; ++e) acc1[m][e][0] = acc1[m][e][1] = acc1[m][e][2] = acc1[m][e][3] = 0.;
+#pragma unroll
+    for (int m = 0; m < M2; ++m)
+        for (int e = 0; e < 2; ++e) acc2[m][e][0] = acc2[m][e][1] = acc2[m][e][2] = acc2[m][e][3] = 0.;
+#pragma unroll
+    for (int m = 0; m < M3; ++m)
+        for (int e = 0; e < 2; ++e) acc3[m][e][0] = acc3[m][e][1] = acc3[m][e][2] = acc3[m][e][3] = 0.;
+#pragma unroll
+    for (int m = 0; m < M4; ++m)
+        for (int e = 0; e < 2; ++e) acc4[m][e][0] = acc4[m][e][1] = acc4[m][e][2] = acc4[m][e][3] = 0.;
+
+    // Same skewed schedule as k_rk4_fused: iteration t, stage 1: q t -> v1 t-2; stage 2:
+    // v1 t-3 -> v2 t-5; stage 3: v2 t-6 -> v3 t-8; stage 4: v3 t-9 -> q' t-11.
+    const int tq0 = z0 - 8, tq1 = zend + 8;
+    const int tend = zend + 11;
+    {
+        const int zo = wrapz(tq0, nk) * iskb;
+#pragma unroll
+        for (int m = 0; m < M0; ++m)
+            *(d2*)&P0[tq0 & 1][2 * min(tid + m * NT, A0P - 1)] = ld2(rin, off0[m], zo);
+        pair_qload<M1>(qv1, rin, wrapz(tq0 - 2, nk) * iskb, q1);
+        pair_qload<M2>(qv2, rin, wrapz(z0 - 8, nk) * iskb, q2);
+        pair_qload<M3>(qv3, rin, wrapz(z0 - 6, nk) * iskb, q3);
+        pair_qload<M4>(qv4, rin, wrapz(z0 - 4, nk) * iskb, q4);
+    }
+    d2 pre[M0];
+    if (tq0 + 1 < tq1) {
+        const int zo = wrapz(tq0 + 1, nk) * iskb;
+#pragma unroll
+        for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+    }
+    __syncthreads();
+
+    auto plane = [&](auto Rc, int t) {
+        constexpr int R = decltype(Rc)::value;
+        const int cur = t & 1, prv = cur ^ 1;
+        __builtin_amdgcn_sched_barrier(0);  // keep planes apart: bounded live ranges
+        if (t + 1 < tq1) {
+#pragma unroll
+            for (int m = 0; m < M0; ++m) *(d2*)&P0[prv][2 * min(tid + m * NT, A0P - 1)] = pre[m];
+            if (t + 2 < tq1) {
+                const int zo = wrapz(t + 2, nk) * iskb;
+#pragma unroll
+                for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (t < tq1) {
+            pair_step<TX, TY, 1, NT, M1, R & 3>(P0[cur], w, acc1, d1);
+#pragma unroll
+            for (int m = 0; m < M1; ++m)
+                *(d2*)&P1[cur][2 * min(tid + m * NT, G1::AO / 2 - 1)] =
+                    d2{fmaT(w.f1, d1[m][0], qv1[m].x), fmaT(w.f1, d1[m][1], qv1[m].y)};
+            pair_qload<M1>(qv1, rin, wrapz(t - 1, nk) * iskb, q1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (t >= tq0 + 5 && t < tq1 + 1) {
+            pair_step<TX, TY, 2, NT, M2, (R + 1) & 3>(P1[prv], w, acc2, d2v);
+#pragma unroll
+            for (int m = 0; m < M2; ++m)
+                *(d2*)&P2[cur][2 * min(tid + m * NT, G2::AO / 2 - 1)] =
+                    d2{fmaT(w.f2, d2v[m][0], qv2[m].x), fmaT(w.f2, d2v[m][1], qv2[m].y)};
+            pair_qload<M2>(qv2, rin, wrapz(t - 4, nk) * iskb, q2);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (t >= tq0 + 10 && t < tq1 + 2) {
+            pair_step<TX, TY, 3, NT, M3, (R + 2) & 3>(P2[prv], w, acc3, d3);
+#pragma unroll
+            for (int m = 0; m < M3; ++m)
+                *(d2*)&P3[cur][2 * min(tid + m * NT, G3::AO / 2 - 1)] =
+                    d2{fmaT(w.f3, d3[m][0], qv3[m].x), fmaT(w.f3, d3[m][1], qv3[m].y)};
+            pair_qload<M3>(qv3, rin, wrapz(t - 7, nk) * iskb, q3);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (t >= tq0 + 15) {
+            pair_step<TX, TY, 4, NT, M4, (R + 3) & 3>(P3[prv], w, acc4, d4);
+            const int zp = t - 11;
+            if (zp >= z0 && zp < zend) {
+                const int zo = zp * oskb;
+#pragma unroll
+                for (int m = 0; m < M4; ++m)
+                    st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], qv4[m].x), fmaT(w.f4, d4[m][1], qv4[m].y)});
+            }
+            pair_qload<M4>(qv4, rin, wrapz(t - 10, nk) * iskb, q4);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    // Stage s runs while its input planes are valid.  The loop is unrolled by 4 so that
+    // the phase R of every plane is a compile-time constant.
+    for (int t = tq0; t < tend; t += 4) {
+        plane(std::integral_constant<int, 0>{}, t);
+        if (t + 1 < tend) plane(std::integral_constant<int, 1>{}, t + 1);
+        if (t + 2 < tend) plane(std::integral_constant<int, 2>{}, t + 2);
+        if (t + 3 < tend) plane(std::integral_constant<int, 3>{}, t + 3);
+    }
+}
+
+}  // namespace
+
+Coefs make_coefs(double nu, double cx, double cy, double cz, double dx, double h)
+{
+    const Coef13 k = make_coef13(nu, cx, cy, cz, dx);
+    Coefs c;
+    const double w[13] = {k.x[0], k.x[1], k.x[2], k.x[3], k.y[0], k.y[1], k.y[2],
+                          k.y[3], k.z[0], k.z[1], k.z[2], k.z[3], k.c};
+    const double f[4] = {0.25 * h, h / 3., 0.5 * h, h};
+    for (int i = 0; i < 13; ++i) {
+        c.wd[i] = w[i];
+        c.wf[i] = (float)w[i];
+    }
+    for (int i = 0; i < 4; ++i) {
+        c.fd[i] = f[i];
+        c.ff[i] = (float)f[i];
+    }
+    return c;
+}
+
+size_t coefs_size() { return sizeof(Coefs); }
+
+void fill_coefs(void* dst, double nu, double cx, double cy, double cz, double dx, double h)
+{
+    *(Coefs*)dst = make_coefs(nu, cx, cy, cz, dx, h);
+}
+
+namespace {
+
+int fill_args(FusedArgs& a, const FusedStep& st, const FusedTile& tile, View in, View out, int isz, int osz)
+{
     a.in = in;
     a.out = out;
+    const long long span_in = (st.ni - 1) + (st.nj - 1) * in.sj + (st.nk - 1) * in.sk + 1;
+    const long long span_out = (st.ni - 1) + (st.nj - 1) * out.sj + (st.nk - 1) * out.sk + 1;
+    const long long bi = span_in * isz, bo = span_out * osz;
+    if (bi >= (1LL << 31) || bo >= (1LL << 31))
+        return fail(SP_ERR_UNSUPPORTED, "fused RK4: one field must span less than 2 GiB");
+    a.in_bytes = (unsigned)bi;
+    a.out_bytes = (unsigned)bo;
     a.ni = st.ni;
     a.nj = st.nj;
     a.nk = st.nk;
     a.kchunk = tile.kchunk > 0 ? min(tile.kchunk, st.nk) : st.nk;
     a.tiles_x = (st.ni + tile.tx - 1) / tile.tx;
     const int tiles_y = (st.nj + tile.ty - 1) / tile.ty;
-    a.h = st.dt;
-    a.cx = st.cx;
-    a.cy = st.cy;
-    a.cz = st.cz;
-    a.dx = st.dx;
-    a.nu_dev = st.nu_dev;
-    a.nu = st.nu;
-    const dim3 grid(a.tiles_x * tiles_y, (st.nk + a.kchunk - 1) / a.kchunk, st.nslices);
+    a.tiles = a.tiles_x * tiles_y;
+    a.chunks = (st.nk + a.kchunk - 1) / a.kchunk;
+    a.c = make_coefs(st.nu, st.cx, st.cy, st.cz, st.dx, st.dt);
+    a.cdev = (const Coefs*)st.coef_dev;
+    const long long nblocks = (long long)a.tiles * a.chunks * st.nslices;
+    if (nblocks > 0x7fffffffLL) return fail(SP_ERR_UNSUPPORTED, "fused RK4: too many blocks");
+    return SP_OK;
+}
+
+unsigned nblocks_of(const FusedArgs& a, int nslices) { return (unsigned)((long long)a.tiles * a.chunks * nslices); }
+
+template <typename SI, typename SO, typename T>
+int launch_typed(hipStream_t s, const FusedStep& st, const FusedTile& tile, View in, View out)
+{
+    FusedArgs a;
+    SP_TRY(fill_args(a, st, tile, in, out, (int)sizeof(SI), (int)sizeof(SO)));
+    const dim3 grid(nblocks_of(a, st.nslices));
 #define SP_FUSED_CASE(TX_, TY_, NT_)                                                             \
     if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                                    \
         hipLaunchKernelGGL((k_rk4_fused<SI, SO, T, TX_, TY_, NT_>), grid, dim3(NT_), 0, s, a);   \
@@ -312,28 +730,72 @@ int launch_typed(hipStream_t s, const FusedStep& st, const FusedTile& tile, View
         return SP_OK;                                                                            \
     }
     SP_FUSED_CASE(32, 32, 1024)
+    SP_FUSED_CASE(32, 24, 1024)
+    SP_FUSED_CASE(24, 24, 1024)
     SP_FUSED_CASE(32, 16, 512)
     SP_FUSED_CASE(16, 16, 256)
-    SP_FUSED_CASE(24, 24, 512)
 #undef SP_FUSED_CASE
     return fail(SP_ERR_UNSUPPORTED, "fused RK4: unsupported tile shape");
+}
+
+int launch_pair(hipStream_t s, const FusedStep& st, const FusedTile& tile, View in, View out)
+{
+    FusedArgs a;
+    SP_TRY(fill_args(a, st, tile, in, out, 8, 8));
+    const dim3 grid(nblocks_of(a, st.nslices));
+#define SP_PAIR_CASE(TX_, TY_, NT_)                                                      \
+    if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                            \
+        hipLaunchKernelGGL((k_rk4_fused_pair<TX_, TY_, NT_>), grid, dim3(NT_), 0, s, a); \
+        SP_HIP(hipGetLastError());                                                       \
+        return SP_OK;                                                                    \
+    }
+    SP_PAIR_CASE(32, 32, 1024)
+    SP_PAIR_CASE(32, 32, 768)
+    SP_PAIR_CASE(32, 32, 512)
+    SP_PAIR_CASE(32, 24, 768)
+    SP_PAIR_CASE(32, 16, 512)
+    SP_PAIR_CASE(16, 16, 256)
+#undef SP_PAIR_CASE
+    return fail(SP_ERR_UNSUPPORTED, "fused RK4 (pair): unsupported tile shape");
+}
+
+// 16-byte pair access needs even x extent, even strides and 16-byte aligned origins.
+bool pair_ok(const FusedStep& st, View in, View out)
+{
+    const auto even = [](long long v) { return (v & 1) == 0; };
+    return even(st.ni) && even(in.sj) && even(in.sk) && even(out.sj) && even(out.sk) && even(in.slice) &&
+           even(out.slice) && ((uintptr_t)in.p & 15) == 0 && ((uintptr_t)out.p & 15) == 0;
 }
 
 }  // namespace
 
 bool fused_tile_supported(const FusedTile& t, int /*compute_f64*/)
 {
-    return (t.tx == 32 && t.ty == 32 && t.nt == 1024) || (t.tx == 32 && t.ty == 16 && t.nt == 512) ||
-           (t.tx == 16 && t.ty == 16 && t.nt == 256) || (t.tx == 24 && t.ty == 24 && t.nt == 512);
+    static const int scalar[][3] = {{32, 32, 1024}, {32, 24, 1024}, {24, 24, 1024}, {32, 16, 512}, {16, 16, 256}};
+    static const int pair[][3] = {{32, 32, 1024}, {32, 32, 768}, {32, 32, 512},
+                                  {32, 24, 768},  {32, 16, 512}, {16, 16, 256}};
+    if (t.variant == SP_KERNEL_PAIR) {
+        for (const auto& sh : pair)
+            if (t.tx == sh[0] && t.ty == sh[1] && t.nt == sh[2]) return true;
+        return false;
+    }
+    for (const auto& sh : scalar)
+        if (t.tx == sh[0] && t.ty == sh[1] && t.nt == sh[2]) return true;
+    return false;
 }
 
-FusedTile default_tile(int ni, int nj, int nk, int nslices, int /*compute_f64*/)
+FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64)
 {
-    FusedTile t{32, 32, 1024, 0};
-    // Aim for at least ~4 blocks per CU (256 CUs) by chunking k.
-    const long long tiles = (long long)((ni + 31) / 32) * ((nj + 31) / 32) * nslices;
-    const long long want = 1024;
-    int chunks = (int)((want + tiles - 1) / tiles);
+    FusedTile t{32, 32, 1024, 0, compute_f64 ? SP_KERNEL_PAIR : SP_KERNEL_SCALAR};
+    if (!compute_f64) {
+        t.tx = 32;
+        t.ty = 16;
+        t.nt = 512;
+    }
+    // Aim for at least one block per CU (256 CUs): chunk k when the tiles alone are too few.
+    const long long blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
+    const long long want = 256;
+    int chunks = (int)((want + blocks - 1) / blocks);
     if (chunks < 1) chunks = 1;
     int kc = (nk + chunks - 1) / chunks;
     if (kc < 16) kc = 16;
@@ -341,10 +803,22 @@ FusedTile default_tile(int ni, int nj, int nk, int nslices, int /*compute_f64*/)
     return t;
 }
 
-int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile, int in_dtype, View in,
+int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile_in, int in_dtype, View in,
                      int out_dtype, View out)
 {
-    if (in_dtype == SP_F64 && out_dtype == SP_F64) return launch_typed<double, double, double>(s, st, tile, in, out);
+    FusedTile tile = tile_in;
+    const bool f64 = in_dtype == SP_F64 && out_dtype == SP_F64;
+    if (f64 && tile.variant == SP_KERNEL_PAIR) {
+        if (pair_ok(st, in, out)) return launch_pair(s, st, tile, in, out);
+        tile = default_tile(st.ni, st.nj, st.nk, st.nslices, 0);  // odd sizes: scalar kernel
+        tile.kchunk = tile_in.kchunk;
+    }
+    if (tile.variant == SP_KERNEL_PAIR) {  // pair shapes are f64-only
+        const int kc = tile.kchunk;
+        tile = default_tile(st.ni, st.nj, st.nk, st.nslices, 0);
+        tile.kchunk = kc;
+    }
+    if (f64) return launch_typed<double, double, double>(s, st, tile, in, out);
     if (in_dtype == SP_F64 && out_dtype == SP_F32) return launch_typed<double, float, float>(s, st, tile, in, out);
     if (in_dtype == SP_F32 && out_dtype == SP_F32) return launch_typed<float, float, float>(s, st, tile, in, out);
     if (in_dtype == SP_F64 && out_dtype == SP_BF16) return launch_typed<double, bf16, float>(s, st, tile, in, out);

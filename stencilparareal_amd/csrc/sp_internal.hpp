@@ -136,15 +136,19 @@ int launch_euler_fused(hipStream_t s, const StagedParams& P, View qin, View qout
 // Fused RK4 step over `nslices` batched slices.  Types by dtype codes.
 struct FusedTile {
     int tx, ty, nt, kchunk;
+    int variant;  // SP_KERNEL_SCALAR or SP_KERNEL_PAIR
 };
 struct FusedStep {
     int ni, nj, nk;
     int nslices;
     double dt;
     double cx, cy, cz, dx;
-    const double* nu_dev;  // per-slice nu for this step (device) or nullptr
-    double nu;             // used when nu_dev == nullptr
+    const void* coef_dev;  // per-slice coefficient records for this step (device) or nullptr
+    double nu;             // used when coef_dev == nullptr
 };
+// Coefficient record of one fused RK4 step (stencil weights and stage factors for dt).
+size_t coefs_size();
+void fill_coefs(void* dst, double nu, double cx, double cy, double cz, double dx, double h);
 int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile, int in_dtype, View in,
                      int out_dtype, View out);
 bool fused_tile_supported(const FusedTile& t, int compute_f64);

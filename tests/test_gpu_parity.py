@@ -91,13 +91,18 @@ def test_euler_bitwise(sp, upwind, cvec):
     assert np.array_equal(a.get(), oc.euler(q0, 0.0, dt, 1))
 
 
+TILES = [("scalar", 32, 32, 1024), ("scalar", 24, 24, 1024), ("scalar", 32, 16, 512), ("scalar", 16, 16, 256),
+         ("pair", 32, 32, 1024), ("pair", 32, 32, 768), ("pair", 32, 32, 512), ("pair", 32, 24, 768),
+         ("pair", 32, 16, 512), ("pair", 16, 16, 256)]
+
+
 @pytest.mark.parametrize("N", [8, 16, 20, 32, 48, 64])
-@pytest.mark.parametrize("tile", [(32, 32, 1024), (24, 24, 512), (32, 16, 512), (16, 16, 256)])
+@pytest.mark.parametrize("tile", TILES, ids=lambda t: "%s-%dx%dx%d" % t)
 def test_fused_rk4_vs_oracle(sp, N, tile):
     nu, dt, steps = 0.01, 1e-3 * (16 / N) ** 2, 6
     q0 = smooth_field(N, N)
     conv = sp.Convection(N, N, N, 1. / N, nu, 0.0, *C3)
-    conv.set_tile(*tile, kchunk=0 if N < 32 else N // 2)
+    conv.set_tile(*tile[1:], kchunk=0 if N < 32 else N // 2, variant=tile[0])
     a, b = conv.field(), conv.field(kind="stella")
     a.set(q0)
     conv.DoRK4(a, b, 0.0, dt, steps)
@@ -105,12 +110,13 @@ def test_fused_rk4_vs_oracle(sp, N, tile):
     assert relerr(b.get(), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("variant", ["scalar", "pair"])
 @pytest.mark.parametrize("kchunk", [0, 7, 16])
-def test_fused_rk4_kchunks_and_nonsquare(sp, kchunk):
+def test_fused_rk4_kchunks_and_nonsquare(sp, kchunk, variant):
     ni, nj, nk = 40, 24, 36
     q0 = rand_field((nk, nj, ni), 5)
     conv = sp.Convection(ni, nj, nk, 1. / 32, 0.01, 0.0, *C3)
-    conv.set_tile(32, 32, 1024, kchunk)
+    conv.set_tile(32, 32, 1024, kchunk, variant=variant)
     a, b = conv.field(), conv.field()
     a.set(q0)
     conv.DoRK4(a, b, 0.0, 2e-4, 3)
@@ -128,6 +134,33 @@ def test_fused_rk4_aliasing_and_zero_steps(sp, steps):
     conv.DoRK4(a, a, 0.1, 1e-3, steps)
     ref = O.OConvection(N, N, N, 1. / N, 0.01, 30.0, *C3).rk4(q0, 0.1, 1e-3, steps) if steps else q0
     assert relerr(a.get(), ref) <= 1e-13
+
+
+def test_pair_and_scalar_kernels_agree_bitwise(sp):
+    # both variants evaluate every point with the same operation sequence
+    N = 48
+    q0 = rand_field((N, N, N), 9)
+    outs = []
+    for variant, tile in (("scalar", (32, 32, 1024)), ("pair", (32, 32, 1024)), ("pair", (32, 16, 512))):
+        conv = sp.Convection(N, N, N, 1. / N, 0.01, 0.0, *C3)
+        conv.set_tile(*tile, kchunk=16, variant=variant)
+        a, b = conv.field(), conv.field()
+        a.set(q0)
+        conv.DoRK4(a, b, 0.0, 1e-4, 3)
+        outs.append(b.get())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_odd_sizes_fall_back_to_scalar(sp):
+    ni, nj, nk = 21, 18, 17
+    q0 = rand_field((nk, nj, ni), 6)
+    conv = sp.Convection(ni, nj, nk, 1. / 21, 0.01, 0.0, *C3)
+    conv.set_tile(32, 32, 1024, 0, variant="pair")
+    a, b = conv.field(), conv.field()
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, 1e-4, 2)
+    ref = O.OConvection(ni, nj, nk, 1. / 21, 0.01, 0.0, *C3).rk4(q0, 0.0, 1e-4, 2)
+    assert relerr(b.get(), ref) <= 1e-12
 
 
 def test_fused_rk4_long_run(sp):
