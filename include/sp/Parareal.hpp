@@ -1,0 +1,95 @@
+// Parareal.hpp -- drop-in C++ Parareal driver over the C ABI.
+//
+// Same shape as the reference template (src/Parareal/Parareal.h:38-289):
+//   Parareal<PropagatorT, FieldType>(propagator, initial, solution, conf)
+//   DoParallel(), DoTimedParallel(std::vector<double>&), DoSerial()
+// The reference runs one slice per MPI rank and takes (tStart, comm); here all slices are
+// the batch index of one GPU, so neither is needed.  The solution field receives the last
+// slice's result (the last rank's q).  Extensions select the coarse propagator
+// (DoEuler as in the reference, or RK4 in f64 / f32 / bf16 storage), a stopping tolerance
+// and hipGraph capture of each iteration.
+#pragma once
+
+#include <vector>
+
+#include "Convection.hpp"
+#include "RuntimeConfiguration.hpp"
+
+namespace sp {
+
+struct PararealOptions {
+    int coarse_kind = SP_COARSE_EULER;
+    int coarse_dtype = SP_F64;
+    int coarse_substeps = 1;
+    double tol = 0.;
+    bool use_graphs = false;
+};
+
+template <typename PropagatorT, typename FieldType>
+class Parareal {
+public:
+    Parareal(PropagatorT& propagator, FieldType& initial, FieldType& solution, const RuntimeConfiguration& conf,
+             const PararealOptions& opt = PararealOptions())
+        : initial_(initial), q_(solution), kmax_(conf.kmax())
+    {
+        sp_parareal_cfg c{};
+        c.slices = conf.timeSlices();
+        c.kmax = conf.kmax();
+        c.end_time = conf.endTime();
+        c.steps_fine = conf.timeStepsFine();
+        c.steps_coarse = conf.timeStepsCoarse();
+        c.coarse_kind = opt.coarse_kind;
+        c.coarse_dtype = opt.coarse_dtype;
+        c.coarse_substeps = opt.coarse_substeps;
+        c.tol = opt.tol;
+        c.async_schedule = conf.async() ? 1 : 0;
+        c.use_graphs = opt.use_graphs ? 1 : 0;
+        check(sp_parareal_create(propagator.handle(), &c, &pr_));
+    }
+    ~Parareal() { (void)sp_parareal_destroy(pr_); }
+    Parareal(const Parareal&) = delete;
+    Parareal& operator=(const Parareal&) = delete;
+
+    void DoParallel()
+    {
+        check(sp_parareal_run(pr_, &initial_.layout(), initial_.device(), &q_.layout(), q_.device(), &report_));
+    }
+
+    // 5*kmax+2 cumulative stamps as in Parareal.h:173-248 (receive and send take no time
+    // on one device; the coarse stamp includes the update).
+    void DoTimedParallel(std::vector<double>& times)
+    {
+        DoParallel();
+        times.assign(5 * kmax_ + 2, 0.);
+        double t = report_.t_init;
+        times[0] = t;
+        for (int k = 0; k < kmax_; ++k) {
+            if (k < report_.iterations) {
+                t += report_.t_fine[k];
+                times[5 * k + 1] = t;
+                times[5 * k + 2] = t;
+                t += report_.t_coarse[k];
+                times[5 * k + 3] = times[5 * k + 4] = times[5 * k + 5] = t;
+            } else {
+                for (int q = 1; q <= 5; ++q) times[5 * k + q] = t;
+            }
+        }
+        times[5 * kmax_ + 1] = report_.t_total;
+    }
+
+    void DoSerial()
+    {
+        check(sp_parareal_serial(pr_, &initial_.layout(), initial_.device(), &q_.layout(), q_.device()));
+    }
+
+    const sp_parareal_report& report() const { return report_; }
+
+private:
+    FieldType& initial_;
+    FieldType& q_;
+    int kmax_;
+    sp_parareal* pr_ = nullptr;
+    sp_parareal_report report_{};
+};
+
+}  // namespace sp

@@ -1,0 +1,191 @@
+// parareal -- single-GPU driver with the run modes and output of the reference's
+// src/Parareal/main.cpp (Compare, Serial, Parallel, Timing), built on the drop-in C++ API
+// (include/sp/*.hpp) over libsp_hip.so.  The slices the reference spreads over MPI ranks
+// are the batch index of one device (--timeSlices).  Extensions: --coarse euler|rk4,
+// --coarseDtype f64|f32|bf16, --tol <t>, --graphs 0|1.  Energy counters of the Cray nodes are not
+// available and not printed.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "sp/Convection.hpp"
+#include "sp/ConvectionField.hpp"
+#include "sp/Parareal.hpp"
+#include "sp/RuntimeConfiguration.hpp"
+
+using namespace sp;
+
+static double wtime()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// exactQ / fillQ, src/Convection/ConvectionSolution.h:8-52 (unit cube, t = 0 here)
+static double exactQ(double nu0, double nufreq, double cx, double cy, double cz, double x, double y, double z,
+                     double t)
+{
+    const double pi = 3.14159265358979;
+    const double a = nufreq > 0 ? nu0 * t - (nu0 / (2. * nufreq)) * (std::cos(nufreq * t) - 1.) : nu0 * t;
+    return std::sin(2. * pi * (x - cx * t)) * std::sin(2. * pi * (y - cy * t)) * std::sin(2. * pi * (z - cz * t)) *
+           std::exp(-12. * pi * pi * a);
+}
+
+static void fillQ(ConvectionField& q, const RuntimeConfiguration& conf, double t)
+{
+    const IJKSize& d = q.calculationDomain();
+    const double dxh = 1. / d.iSize() / 2., dyh = 1. / d.jSize() / 2., dzh = 1. / d.kSize() / 2.;
+    for (int i = 0; i < d.iSize(); ++i)
+        for (int j = 0; j < d.jSize(); ++j)
+            for (int k = 0; k < d.kSize(); ++k)
+                q(i, j, k) = exactQ(conf.nu0(), conf.nufreq(), conf.cx(), conf.cy(), conf.cz(), (2 * i + 1) * dxh,
+                                    (2 * j + 1) * dyh, (2 * k + 1) * dzh, t);
+}
+
+// computeErrorReference, src/Parareal/main.cpp:98-139
+static double computeErrorReference(const ConvectionField& field, const ConvectionField& reference)
+{
+    field.SynchronizeHostStorage();
+    reference.SynchronizeHostStorage();
+    const IJKSize& s = field.calculationDomain();
+    double error = 0., maxfield = 0.;
+    for (int i = 0; i < s.iSize(); ++i)
+        for (int j = 0; j < s.jSize(); ++j)
+            for (int k = 0; k < s.kSize(); ++k) {
+                const double e = std::abs(field(i, j, k) - reference(i, j, k));
+                error = std::max(error, e);
+                maxfield = std::max(std::abs(field(i, j, k)), maxfield);
+            }
+    return error / maxfield;
+}
+
+int main(int argc, char** argv)
+{
+    std::cout << "Initialization...\n" << std::endl;
+    RuntimeConfiguration conf(argc, argv);
+    PararealOptions opt;
+    for (int a = 1; a < argc; ++a) {
+        const std::string k = argv[a];
+        const std::string v = a + 1 < argc ? argv[a + 1] : "";
+        if (k == "--coarse") opt.coarse_kind = v == "rk4" ? SP_COARSE_RK4 : SP_COARSE_EULER;
+        if (k == "--coarseDtype") opt.coarse_dtype = v == "f32" ? SP_F32 : v == "bf16" ? SP_BF16 : SP_F64;
+        if (k == "--tol") opt.tol = std::atof(v.c_str());
+        if (k == "--graphs") opt.use_graphs = v != "0";
+    }
+
+    std::cout << "Running with:\n"
+              << " - initial diffusion coefficient: " << conf.nu0() << "\n"
+              << " - frequence of diffusion coefficient: " << conf.nufreq() << "\n"
+              << " - advection velocity in x: " << conf.cx() << "\n"
+              << " - advection velocity in y: " << conf.cy() << "\n"
+              << " - advection velocity in z: " << conf.cz() << "\n"
+              << " - spatial discretization step: " << conf.dx() << "\n"
+              << " - endtime: " << conf.endTime() << "\n"
+              << " - number of time slices: " << conf.timeSlices() << "\n"
+              << " - time slice size: " << conf.timeSliceSize() << "\n"
+              << " - CFL fine: " << conf.cflFine() << "\n"
+              << " - CFL coarse: " << conf.cflCoarse() << "\n"
+              << " - timestep size fine: " << conf.dtFine() << "\n"
+              << " - timestep size coarse: " << conf.dtCoarse() << "\n"
+              << " - timesteps per slice fine propagator: " << conf.timeStepsFinePerTimeSlice() << "\n"
+              << " - timesteps per slice coarse propagator: " << conf.timeStepsCoarsePerTimeSlice() << "\n"
+              << " - parareal iterations: " << conf.kmax() << "\n"
+              << " - asynchronous communications: " << (conf.async() ? "Enabled" : "Disabled") << "\n"
+              << " - intermediate fields in mat files: " << (conf.mat() ? "Yes" : "No") << "\n"
+              << std::endl;
+
+    try {
+        IJKSize domain;
+        domain.Init(conf.gridSize(), conf.gridSize(), conf.gridSize());
+        KBoundary kboundary;
+        kboundary.Init(-2, 2);
+        ConvectionField q, qinitial;
+        q.Init("q", domain, kboundary);
+        qinitial.Init("qinitial", domain, kboundary);
+        Convection convection(conf.gridSize(), conf.gridSize(), conf.gridSize(), conf.dx(), conf.nu0(),
+                              conf.nufreq(), conf.cx(), conf.cy(), conf.cz());
+        Parareal<Convection, ConvectionField> parareal(convection, qinitial, q, conf, opt);
+
+        if (conf.mode() == ModeCompare) {
+            // cost ratio of the propagators on 4 slices' steps (main.cpp:197-210)
+            const int tauSamples = 4;
+            check(sp_ctx_sync(context()));
+            double tauF = wtime();
+            convection.DoRK4(qinitial, qinitial, 0., conf.dtFine(), tauSamples * conf.timeStepsFinePerTimeSlice());
+            check(sp_ctx_sync(context()));
+            tauF = wtime() - tauF;
+            double tauG = wtime();
+            convection.DoEuler(qinitial, qinitial, 0., conf.dtCoarse(),
+                               tauSamples * conf.timeStepsCoarsePerTimeSlice());
+            check(sp_ctx_sync(context()));
+            tauG = wtime() - tauG;
+            const double tauRatio = tauG / tauF;
+            const double Nit_Np = static_cast<double>(conf.kmax()) / conf.timeSlices();
+            const double maxSpeedup = 1. / (tauRatio * (1. + Nit_Np) + Nit_Np);
+
+            fillQ(qinitial, conf, 0.);
+            qinitial.SynchronizeDeviceStorage();
+            double eserial = wtime();
+            parareal.DoSerial();
+            check(sp_ctx_sync(context()));
+            eserial = wtime() - eserial;
+            ConvectionField qreference = q;
+
+            fillQ(qinitial, conf, 0.);
+            qinitial.SynchronizeDeviceStorage();
+            double eparallel = wtime();
+            parareal.DoParallel();
+            check(sp_ctx_sync(context()));
+            eparallel = wtime() - eparallel;
+
+            const double e = computeErrorReference(q, qreference);
+            std::cout << "\n"
+                      << "Serial run time: " << eserial << "\n"
+                      << "Parallel run time: " << eparallel << "\n"
+                      << "Speedup: " << eserial / eparallel << "\n"
+                      << "Maximal speedup: " << maxSpeedup << "\n"
+                      << "Error at end: " << e << "\n"
+                      << std::endl;
+        } else if (conf.mode() == ModeSerial) {
+            fillQ(qinitial, conf, 0.);
+            qinitial.SynchronizeDeviceStorage();
+            double e = wtime();
+            parareal.DoSerial();
+            check(sp_ctx_sync(context()));
+            e = wtime() - e;
+            std::cout << "\n" << "Serial run time: " << e << "\n" << std::endl;
+        } else if (conf.mode() == ModeParallel) {
+            fillQ(qinitial, conf, 0.);
+            qinitial.SynchronizeDeviceStorage();
+            parareal.DoSerial();
+            std::cout << " -- The serial computation is done\n";
+            ConvectionField qreference = q;
+            double e = wtime();
+            parareal.DoParallel();
+            check(sp_ctx_sync(context()));
+            e = wtime() - e;
+            std::cout << " -- The parallel computation is done\n";
+            const double error = computeErrorReference(q, qreference);
+            std::cout << std::endl;
+            std::printf("Parallel run time: %f s\n", e);
+            std::printf("Error of parareal: %.4e\n", error);
+            std::cout << std::endl;
+        } else {
+            fillQ(qinitial, conf, 0.);
+            qinitial.SynchronizeDeviceStorage();
+            std::vector<double> times;
+            parareal.DoTimedParallel(times);
+            std::cout << "\nTimes:\n";
+            for (size_t i = 0; i < times.size(); ++i)
+                std::cout << std::scientific << std::setprecision(6) << times[i] << "   " << "\n";
+        }
+    } catch (const std::exception& ex) {
+        std::cerr << ex.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}
