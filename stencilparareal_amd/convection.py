@@ -13,11 +13,12 @@ class Convection:
 
     upwind: Advection2 branch of DoEuler (True = the reference build's default,
     CMakeLists.txt:101-105).  path: "fused" (one kernel per RK4 step) or "staged"
-    (the reference's 4 RHS + 3 Euler + 1 RK sweeps, bit-for-bit its operation order).
+    (the reference's 4 RHS + 3 Euler + 1 RK sweeps, bit-for-bit its operation order);
+    None keeps the library's choice (fused, or staged when ksize < 8).
     """
 
     def __init__(self, isize, jsize, ksize, dx, nu0, nufreq, cx, cy, cz, *, upwind=True,
-                 path="fused", ctx: Context | None = None, device: int = 0):
+                 path=None, ctx: Context | None = None, device: int = 0):
         self.ctx = ctx or Context.default(device)
         self.sizes = (isize, jsize, ksize)
         self.dx, self.nu0, self.nufreq = dx, nu0, nufreq
@@ -26,7 +27,8 @@ class Convection:
         L.call("sp_prop_create", self.ctx.h, isize, jsize, ksize, dx, nu0, nufreq, cx, cy, cz, C.byref(h))
         self.h = h
         self.set_upwind(upwind)
-        self.set_path(path)
+        if path is not None:
+            self.set_path(path)
 
     def set_upwind(self, upwind: bool):
         L.call("sp_prop_set_upwind", self.h, 1 if upwind else 0)

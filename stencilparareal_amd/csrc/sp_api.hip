@@ -377,6 +377,9 @@ int sp_prop_create(sp_ctx* c, int isize, int jsize, int ksize, double dx, double
     p->cx = cx;
     p->cy = cy;
     p->cz = cz;
+    // the fused kernel's k stream needs nk >= kFusedMinK; thinner grids run the
+    // reference-order sweeps
+    if (ksize < kFusedMinK) p->path = SP_PATH_STAGED;
     *out = p;
     return SP_OK;
 }
@@ -409,6 +412,8 @@ int sp_prop_set_path(sp_prop* p, int path)
 {
     SP_CHECK(p, SP_ERR_INVALID, "sp_prop_set_path: null propagator");
     SP_CHECK(path == SP_PATH_FUSED || path == SP_PATH_STAGED, SP_ERR_INVALID, "sp_prop_set_path: unknown path");
+    SP_CHECK(path == SP_PATH_STAGED || p->nk >= kFusedMinK, SP_ERR_UNSUPPORTED,
+             "sp_prop_set_path: the fused path needs ksize >= " + std::to_string(kFusedMinK));
     p->path = path;
     return SP_OK;
 }

@@ -808,6 +808,9 @@ int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile_i
 {
     FusedTile tile = tile_in;
     const bool f64 = in_dtype == SP_F64 && out_dtype == SP_F64;
+    if (st.nk < kFusedMinK)
+        return fail(SP_ERR_UNSUPPORTED, "fused RK4: ksize " + std::to_string(st.nk) + " < " +
+                                            std::to_string(kFusedMinK) + " (use the staged path)");
     if (f64 && tile.variant == SP_KERNEL_PAIR) {
         if (pair_ok(st, in, out)) return launch_pair(s, st, tile, in, out);
         tile = default_tile(st.ni, st.nj, st.nk, st.nslices, 0);  // odd sizes: scalar kernel

@@ -168,5 +168,7 @@ int launch_fill_exact(hipStream_t s, const sp_layout& l, void* base, double nu0,
 int launch_fill_gaussians(hipStream_t s, const sp_layout& l, void* base, int ng, const double* gdev);
 
 constexpr int kReduceBlocks = 512;
+// Smallest k extent the fused RK4 kernel streams (its plane wrap covers [-3nk, 4nk)).
+constexpr int kFusedMinK = 8;
 
 }  // namespace sp

@@ -163,6 +163,25 @@ def test_odd_sizes_fall_back_to_scalar(sp):
     assert relerr(b.get(), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("nk", [2, 4, 7])
+def test_thin_k_runs_reference_order_path(sp, nk):
+    # k extents below 8 are outside the fused kernel's plane wrap: the propagator starts on
+    # the reference-order path (bit-exact) and refuses the fused one with an error
+    from stencilparareal_amd._lib import SPError
+    ni, nj = 12, 10
+    q0 = rand_field((nk, nj, ni), nk)
+    conv = sp.Convection(ni, nj, nk, 1. / 12, 0.01, 0.0, *C3)
+    a, b = conv.field(), conv.field()
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, 1e-4, 3)
+    ref = O.OConvection(ni, nj, nk, 1. / 12, 0.01, 0.0, *C3).rk4(q0, 0.0, 1e-4, 3)
+    assert np.array_equal(b.get(), ref)
+    with pytest.raises(SPError):
+        conv.set_path("fused")
+    with pytest.raises(SPError):  # mixed precision exists only on the fused path
+        conv.DoRK4(a, conv.field(dtype="f32"), 0.0, 1e-4, 1)
+
+
 def test_fused_rk4_long_run(sp):
     N, steps = 32, 300
     q0 = O.fill_gaussians(N, [0.3, 0.7], [0.5, 0.2], [0.6, 0.9], [0.08, 0.05], [1.0, 0.7])
