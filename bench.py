@@ -56,6 +56,77 @@ def cpu_baseline(w, budget_s=12.0):
             "sample": f"serial fine RK4 (oracle, reference OpenMP structure), {N}^3, {n} steps, {t:.2f} s"}
 
 
+def measure_batched_step(conv, init, conf, S, reps=50, copy_check=False):
+    """Average duration of one batched fused RK4 step over S slices (sp_do_rk4_batched,
+    steps = 1), timed with HIP events on the stream the library launches on (torch's
+    current stream, which the context follows).  Returns (ms, algorithmic bytes)."""
+    import torch
+    from stencilparareal_amd import _lib as L
+    N = init.layout.ni
+    lay = init.layout
+    n = N ** 3
+    bin_ = torch.empty(S * n, dtype=torch.float64, device="cuda")
+    bout = torch.empty_like(bin_)
+    for b in range(S):
+        bin_[b * n:(b + 1) * n].copy_(init.interior().reshape(-1))
+    conv.ctx.sync_stream()
+    dt_f = conf.dtFine()
+
+    def step(src, dst):
+        L.call("sp_do_rk4_batched", conv.h, S, C_ptr(lay), ctypes_vp(src), C_ptr(lay), ctypes_vp(dst), n, 0.0,
+               dt_f, 1)
+
+    for _ in range(3):
+        step(bin_, bout)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for r in range(reps):
+        step(*((bin_, bout) if r % 2 == 0 else (bout, bin_)))
+    ev1.record()
+    ev1.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / reps
+    if copy_check:
+        # a plain streaming kernel over the same bytes (read S*N^3 doubles, write as many):
+        # the scale for the PMC byte counters
+        for _ in range(reps):
+            torch.mul(bin_, 1.0, out=bout)
+        torch.cuda.synchronize()
+    return k_ms, 16.0 * S * n  # FP64 state read once + written once per lattice update
+
+
+def committed_traffic():
+    """HBM bytes per launch of the batched step, from the committed PMC summary (two
+    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy kernel)."""
+    path = os.path.join(ROOT, "profiles", "pmc_rk4_fused_pair_16x128cubed_fp64.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def roofline_only(a):
+    """Only the dominant kernel (and the copy kernel used as the PMC scale), for profilers."""
+    import torch
+    import stencilparareal_amd as sp
+    from stencilparareal_amd.config import RuntimeConfiguration, workload
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    w = workload(2)
+    N, S = w.N, w.slices
+    conf = RuntimeConfiguration(gridSize=N, endTime=w.T, timeStepsFine=w.stepsF, timeStepsCoarse=w.stepsG,
+                                nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=w.kmax, async_=False,
+                                timeSlices=S)
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
+    init = conv.field("qinitial")
+    g = w.gauss
+    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    k_ms, alg = measure_batched_step(conv, init, conf, S, reps=a.reps, copy_check=True)
+    print(json.dumps({"kernel_avg_ms": round(k_ms, 4), "bytes_per_launch": alg, "tile": conv.get_tile(S, True),
+                      "copy_bytes_read": alg / 2, "copy_bytes_written": alg / 2, "reps": a.reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,14 +135,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-compare", action="store_true", help="skip comparator runs")
     ap.add_argument("--graphs", type=int, default=1, help="capture each iteration as a hipGraph")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the batched RK4 step and a copy kernel of the same size (for profilers)")
+    ap.add_argument("--reps", type=int, default=20, help="launches per kernel with --roofline-only")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the time slices are the batch "
               "index of a single device; multi-GPU slice sharding is not part of this build.",
               file=sys.stderr)
         sys.exit(2)
+    if a.roofline_only:
+        return roofline_only(a)
 
-    import numpy as np
     import torch
 
     import stencilparareal_amd as sp
@@ -129,30 +204,10 @@ def main():
 
     # roofline of the dominant kernel: one batched fused RK4 step over all 16 slices
     log("roofline")
-    lay = init.layout
-    n = N ** 3
-    bin_ = torch.empty(S * n, dtype=torch.float64, device="cuda")
-    bout = torch.empty_like(bin_)
-    for b in range(S):
-        bin_[b * n:(b + 1) * n].copy_(init.interior().reshape(-1))
-    conv.ctx.sync_stream()
-    dt_f = conf.dtFine()
-    for _ in range(3):
-        L.call("sp_do_rk4_batched", conv.h, S, C_ptr(lay), ctypes_vp(bin_), C_ptr(lay), ctypes_vp(bout), n, 0.0,
-               dt_f, 1)
-    reps = 50
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for r in range(reps):
-        src, dst = (bin_, bout) if r % 2 == 0 else (bout, bin_)
-        L.call("sp_do_rk4_batched", conv.h, S, C_ptr(lay), ctypes_vp(src), C_ptr(lay), ctypes_vp(dst), n, 0.0,
-               dt_f, 1)
-    ev1.record()
-    ev1.synchronize()
-    k_ms = ev0.elapsed_time(ev1) / reps
-    alg_bytes = 16.0 * S * n  # read + write of the FP64 state per lattice update
+    k_ms, alg_bytes = measure_batched_step(conv, init, conf, S, reps=50)
     achieved = alg_bytes / (k_ms / 1e3) / 1e9
-    del bin_, bout
+    tile = conv.get_tile(S, True)
+    traffic = committed_traffic()
 
     extra = {}
     if not a.no_compare:
@@ -195,9 +250,16 @@ def main():
         "rel_l2_err_per_iterate": [float(f"{e:.4e}") for e in errs_l2],
         "rel_linf_err_per_iterate": [float(f"{e:.4e}") for e in errs_inf],
         "fine_glups_serial": round(N ** 3 * conf.timeStepsFine / t_serial / 1e9, 3),
+        # with graphs an iteration (batched F + G chain + correction) is one launch, so its
+        # split is the batched step's kernel time x fine steps per slice vs the remainder
+        "phase_s": {"init_coarse": round(rep.t_init, 4),
+                    "iteration": [round(f + c, 4) for f, c in zip(rep.t_fine[:rep.iterations],
+                                                                   rep.t_coarse[:rep.iterations])],
+                    "fine_batched_kernel_est": round(k_ms * tsF / 1e3, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_rk4_fused<double,double,double> over 16 slices",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"k_rk4_fused_{tile['variant']}<{tile['tx']},{tile['ty']},{tile['nthreads']}> "
+                               f"(FP64, {S} slices of {N}^3, kchunk {tile['kchunk']})",
                      "avg_launch_ms": round(k_ms, 4), "bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

@@ -116,6 +116,13 @@ int sp_prop_set_path(sp_prop* p, int path);
  * 32x16x512, 16x16x256}, pair {32x32x1024, 32x32x768, 32x32x512, 32x24x768, 32x16x512,
  * 16x16x256}. */
 int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int kchunk);
+/* The fused-kernel tile sp_do_rk4 (nslices = 1) or sp_do_rk4_batched (nslices > 1) will
+ * use for FP64 compute (f64 = 1) or reduced-precision storage (f64 = 0): the one set with
+ * sp_prop_set_tile, else the built-in choice for this size.  Output pointers may be NULL.
+ * Fields whose sizes or alignment the pair kernel cannot take run the scalar 32x16x512
+ * kernel instead (no counterpart in the reference; reports what is launched). */
+int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* tx, int* ty, int* nthreads,
+                     int* kchunk);
 
 /* DoRK4: `steps` RK4 steps of size dt from tStart; step i runs at tStart + i*dt.
  * in == out is allowed.  steps == 0 leaves out untouched.  in and out may have

@@ -45,6 +45,14 @@ class Convection:
             v = {"scalar": L.SP_KERNEL_SCALAR, "pair": L.SP_KERNEL_PAIR, None: L.SP_KERNEL_SCALAR}[variant]
         L.call("sp_prop_set_tile", self.h, v, tx, ty, nthreads, kchunk)
 
+    def get_tile(self, nslices: int = 1, f64: bool = True) -> dict:
+        """The fused-kernel tile a call with `nslices` slices launches."""
+        v = [C.c_int() for _ in range(5)]
+        L.call("sp_prop_get_tile", self.h, nslices, 1 if f64 else 0, *[C.byref(x) for x in v])
+        var, tx, ty, nt, kc = (x.value for x in v)
+        return {"variant": {L.SP_KERNEL_SCALAR: "scalar", L.SP_KERNEL_PAIR: "pair"}.get(var, var), "tx": tx,
+                "ty": ty, "nthreads": nt, "kchunk": kc}
+
     def computenu(self, t: float) -> float:
         import math
         return self.nu0 * (1. + .5 * math.sin(self.nufreq * t))

@@ -434,6 +434,21 @@ int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int 
     return SP_OK;
 }
 
+int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* tx, int* ty, int* nthreads,
+                     int* kchunk)
+{
+    SP_CHECK(p, SP_ERR_INVALID, "sp_prop_get_tile: null propagator");
+    SP_CHECK(nslices >= 1, SP_ERR_INVALID, "sp_prop_get_tile: nslices < 1");
+    FusedTile t = pick_tile(p, nslices, f64 ? 1 : 0);
+    if (!f64 && t.variant == SP_KERNEL_PAIR) t = FusedTile{32, 16, 512, t.kchunk, SP_KERNEL_SCALAR};
+    if (variant) *variant = t.variant;
+    if (tx) *tx = t.tx;
+    if (ty) *ty = t.ty;
+    if (nthreads) *nthreads = t.nt;
+    if (kchunk) *kchunk = t.kchunk;
+    return SP_OK;
+}
+
 static int rk4_common(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
                       double tStart, double dt, int steps, bool f64_only, const char* name)
 {

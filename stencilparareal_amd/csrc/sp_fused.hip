@@ -786,15 +786,19 @@ bool fused_tile_supported(const FusedTile& t, int /*compute_f64*/)
 
 FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64)
 {
-    FusedTile t{32, 32, 1024, 0, compute_f64 ? SP_KERNEL_PAIR : SP_KERNEL_SCALAR};
-    if (!compute_f64) {
-        t.tx = 32;
+    // Measured on MI355X (16 x 128^3 and 1 x 128^3 FP64): the spill-free x-pair kernel with
+    // 32x32 tiles of 512 threads is fastest once the tiles alone give one block per CU;
+    // below that, 32x16 tiles chunked in k fill the chip better.  FP32 / BF16 storage runs
+    // the scalar kernel.
+    const long long want = 256;  // one block per CU
+    FusedTile t{32, 32, 512, 0, compute_f64 ? SP_KERNEL_PAIR : SP_KERNEL_SCALAR};
+    if (!compute_f64) t.nt = 1024;
+    long long blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
+    if (blocks < want) {
         t.ty = 16;
         t.nt = 512;
+        blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
     }
-    // Aim for at least one block per CU (256 CUs): chunk k when the tiles alone are too few.
-    const long long blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
-    const long long want = 256;
     int chunks = (int)((want + blocks - 1) / blocks);
     if (chunks < 1) chunks = 1;
     int kc = (nk + chunks - 1) / chunks;
@@ -811,15 +815,14 @@ int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile_i
     if (st.nk < kFusedMinK)
         return fail(SP_ERR_UNSUPPORTED, "fused RK4: ksize " + std::to_string(st.nk) + " < " +
                                             std::to_string(kFusedMinK) + " (use the staged path)");
-    if (f64 && tile.variant == SP_KERNEL_PAIR) {
-        if (pair_ok(st, in, out)) return launch_pair(s, st, tile, in, out);
-        tile = default_tile(st.ni, st.nj, st.nk, st.nslices, 0);  // odd sizes: scalar kernel
-        tile.kchunk = tile_in.kchunk;
-    }
-    if (tile.variant == SP_KERNEL_PAIR) {  // pair shapes are f64-only
-        const int kc = tile.kchunk;
-        tile = default_tile(st.ni, st.nj, st.nk, st.nslices, 0);
-        tile.kchunk = kc;
+    if (f64 && tile.variant == SP_KERNEL_PAIR && pair_ok(st, in, out)) return launch_pair(s, st, tile, in, out);
+    if (tile.variant == SP_KERNEL_PAIR) {
+        // odd sizes, unaligned views or reduced-precision storage: the scalar kernel with
+        // its fastest FP64 shape and the caller's k chunk
+        tile.tx = 32;
+        tile.ty = 16;
+        tile.nt = 512;
+        tile.variant = SP_KERNEL_SCALAR;
     }
     if (f64) return launch_typed<double, double, double>(s, st, tile, in, out);
     if (in_dtype == SP_F64 && out_dtype == SP_F32) return launch_typed<double, float, float>(s, st, tile, in, out);
