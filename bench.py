@@ -47,7 +47,7 @@ def cpu_baseline(w, budget_s=12.0):
     t0 = O.wtime()
     conv.rk4(q0, 0.0, dt, 2)
     t_probe = max(O.wtime() - t0, 1e-3)
-    n = max(2, min(400, int(budget_s / (t_probe / 2))))
+    n = max(2, min(20000, int(budget_s / (t_probe / 2))))
     t0 = O.wtime()
     conv.rk4(q0, 0.0, dt, n)
     t = O.wtime() - t0

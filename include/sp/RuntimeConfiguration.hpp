@@ -59,7 +59,7 @@ public:
                 }
             }
         }
-        // RuntimeConfiguration.cpp:259-277
+        // RuntimeConfiguration.cpp:133-151
         if (timeStepsFine_ % timeSlices_ != 0) {
             timeStepsFine_ = (timeStepsFine_ / timeSlices_ + 1) * timeSlices_;
             std::cout << "Warning: timeStepsFine is not a multiple of timeSlices\n"

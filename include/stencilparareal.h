@@ -174,7 +174,7 @@ int sp_fill_gaussians(sp_ctx* ctx, const sp_layout* l, void* base, int ng, const
  * one GPU: the fine propagation of all slices runs as one launch per step, the
  * coarse correction chain runs slice after slice, the update is fused with an
  * increment norm.  Step totals are rounded up to multiples of S as in
- * RuntimeConfiguration.cpp:259-277; dt = T / total steps. */
+ * RuntimeConfiguration.cpp:133-151; dt = T / total steps. */
 typedef struct sp_parareal_cfg {
     int slices;          /* S (the reference's MPI world size) */
     int kmax;            /* maximum iterations */
@@ -186,7 +186,7 @@ typedef struct sp_parareal_cfg {
     int coarse_substeps; /* >= 1: each coarse step is this many sub-steps */
     double tol;          /* stop once the relative L2 increment <= tol; 0: run kmax */
     int async_schedule;  /* 1: fine-step times of the reference's async mode */
-    int use_graphs;      /* 1: capture each iteration's fine sweep as a hipGraph */
+    int use_graphs;      /* 1: capture each iteration (fine sweep, coarse chain, updates) as one hipGraph */
 } sp_parareal_cfg;
 
 #define SP_MAX_ITERS 64

@@ -40,7 +40,7 @@ class RuntimeConfiguration:
         self.round_steps()
 
     def round_steps(self):
-        """RuntimeConfiguration.cpp:259-277: totals rounded up to multiples of the slices."""
+        """RuntimeConfiguration.cpp:133-151: totals rounded up to multiples of the slices."""
         S = self.timeSlices
         if self.timeStepsFine % S != 0:
             self.timeStepsFine = (self.timeStepsFine // S + 1) * S

@@ -839,7 +839,7 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
     pr->cfg = *cfg;
     pr->S = cfg->slices;
     pr->T = cfg->end_time;
-    // RuntimeConfiguration.cpp:259-277: totals rounded up to multiples of the slice count
+    // RuntimeConfiguration.cpp:133-151: totals rounded up to multiples of the slice count
     int sf = cfg->steps_fine, sc = cfg->steps_coarse;
     if (sf % pr->S != 0) sf = (sf / pr->S + 1) * pr->S;
     if (sc % pr->S != 0) sc = (sc / pr->S + 1) * pr->S;

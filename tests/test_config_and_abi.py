@@ -2,7 +2,7 @@
 
 * RuntimeConfiguration reproduces the banner values the reference printed in its own
   runs (jobs/paperJobs/*/run_*_0.log), including the step rounding of
-  RuntimeConfiguration.cpp:259-277.
+  RuntimeConfiguration.cpp:133-151.
 * libsp_hip.so loads and exports every function include/stencilparareal.h declares,
   and the header compiles as C and as C++.
 * The product package does not reference the oracle.
