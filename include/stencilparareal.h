@@ -113,8 +113,7 @@ int sp_prop_set_path(sp_prop* p, int path);
 /* Fused-kernel choice: variant SP_KERNEL_*, tile tx x ty columns, nthreads per block,
  * kchunk planes per block (0: whole k extent).  variant SP_KERNEL_AUTO restores the
  * built-in choice.  Compiled shapes: scalar {32x32x1024, 32x24x1024, 24x24x1024,
- * 32x16x512, 16x16x256}, pair {32x32x1024, 32x32x768, 32x32x512, 32x24x768, 32x16x512,
- * 16x16x256}. */
+ * 32x16x512, 16x16x256}, pair {32x32x512, 32x16x512, 16x16x256}. */
 int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int kchunk);
 /* The fused-kernel tile sp_do_rk4 (nslices = 1) or sp_do_rk4_batched (nslices > 1) will
  * use for FP64 compute (f64 = 1) or reduced-precision storage (f64 = 0): the one set with

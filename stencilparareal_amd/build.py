@@ -6,8 +6,8 @@
 Each .hip file is compiled to an object under stencilparareal_amd/_build/ and the
 objects are linked into stencilparareal_amd/libsp_hip.so.  The reference-order
 sweeps (sp_staged.hip, sp_misc.hip) are compiled with -ffp-contract=off.  The C++
-driver stencilparareal_amd/bin/parareal (host/parareal_main.cpp over the C++ API in
-include/sp/) is then built with g++ against that library.
+drivers stencilparareal_amd/bin/parareal and bin/convection (host/*_main.cpp over the
+C++ API in include/sp/) are then built with g++ against that library.
 """
 from __future__ import annotations
 
@@ -22,8 +22,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsp_hip.so")
-DRIVER_SRC = os.path.join(HERE, "host", "parareal_main.cpp")
-DRIVER = os.path.join(HERE, "bin", "parareal")
+# C++ drivers over include/sp/: executable name -> source
+DRIVERS = {"parareal": "parareal_main.cpp", "convection": "convection_main.cpp"}
 ARCH = os.environ.get("SP_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = {
@@ -89,25 +89,31 @@ def build(force: bool = False, verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
-def build_driver(force: bool = False, verbose: bool = False) -> str:
-    """g++ build of the `parareal` executable (rpath to the library one directory up)."""
+def build_driver(force: bool = False, verbose: bool = False) -> list[str]:
+    """g++ build of the driver executables in bin/ (rpath to the library one directory up)."""
     hdr_dir = os.path.join(ROOT, "include")
-    deps = [DRIVER_SRC, LIB] + [os.path.join(hdr_dir, "sp", f) for f in os.listdir(os.path.join(hdr_dir, "sp"))]
-    if not (force or _stale(DRIVER, deps)):
-        return DRIVER
-    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
-    cxx = os.environ.get("CXX", "g++")
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", hdr_dir, DRIVER_SRC, "-L", HERE, "-lsp_hip",
-           "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("driver build failed:\n" + r.stdout)
-    if r.stdout.strip():
-        print(r.stdout, file=sys.stderr)
-    os.replace(DRIVER + ".tmp", DRIVER)
-    return DRIVER
+    hdrs = [os.path.join(hdr_dir, "sp", f) for f in os.listdir(os.path.join(hdr_dir, "sp"))]
+    hdrs.append(os.path.join(hdr_dir, "stencilparareal.h"))
+    out = []
+    for exe, src in DRIVERS.items():
+        src = os.path.join(HERE, "host", src)
+        target = os.path.join(HERE, "bin", exe)
+        out.append(target)
+        if not (force or _stale(target, [src, LIB] + hdrs)):
+            continue
+        os.makedirs(os.path.dirname(target), exist_ok=True)
+        cxx = os.environ.get("CXX", "g++")
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", hdr_dir, src, "-L", HERE, "-lsp_hip",
+               "-Wl,-rpath,$ORIGIN/..", "-o", target + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{exe} build failed:\n" + r.stdout)
+        if r.stdout.strip():
+            print(r.stdout, file=sys.stderr)
+        os.replace(target + ".tmp", target)
+    return out
 
 
 def _wait(procs):

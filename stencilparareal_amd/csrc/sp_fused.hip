@@ -433,7 +433,7 @@ __device__ __forceinline__ int pair_centre(int g)
 
 template <int TX, int TY, int S, int NT, int M, int R>
 __device__ __forceinline__ void pair_step(const double* __restrict__ P, const StarW<double>& w, double (&acc)[M][2][4],
-                                          double (&done)[M][2])
+                                          double (&done)[M][2], d2 (&cen)[M])
 {
     using G = Geo<TX, TY, S>;
     constexpr int WI = G::WI, AP = G::AO / 2;
@@ -445,6 +445,7 @@ __device__ __forceinline__ void pair_step(const double* __restrict__ P, const St
         // x group first (centres and x neighbours), then the y group: at most 4 pairs of
         // loaded values are live at a time
         const d2 C = *(const d2*)c, Lx = *(const d2*)(c - 2), Rx = *(const d2*)(c + 2);
+        cen[m] = C;
         double ax[2];
         {
             const double u[2] = {C.x, C.y};
@@ -502,6 +503,32 @@ __device__ __forceinline__ void pair_qload(d2 (&qv)[M], __amdgpu_buffer_rsrc_t r
     for (int m = 0; m < M; ++m) qv[m] = ld2(r, q[m], zo);
 }
 
+// Index of pair g of stage S's output region inside a plane of region stage S+d
+// (d = 0: the same region; d = 1: the next, smaller one, or `dump` when the pair lies in
+// the halo that the next stage does not compute).
+template <int TX, int TY, int S, int NT, int M>
+__device__ __forceinline__ void pair_qidx(int (&same)[M], int (&next)[M])
+{
+    using G = Geo<TX, TY, S>;
+    constexpr int WP = G::WO / 2, AP = G::AO / 2, WN = G::WO - 4, AN = WN * (G::HOo - 4);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int g = min((int)threadIdx.x + m * NT, AP - 1);
+        const int ax = 2 * (g % WP), by = g / WP;
+        same[m] = by * G::WO + ax;
+        const bool in = ax >= 2 && ax < G::WO - 2 && by >= 2 && by < G::HOo - 2;
+        next[m] = in ? (by - 2) * WN + ax - 2 : AN;  // AN: a dump pair past the plane
+    }
+}
+
+// FP64 RK4 step, two x-adjacent columns per lane.  The stage updates v_s = q + f_s L v_{s-1}
+// need q at each stage's output plane, which lags the q plane being loaded by 2, 5, 8 and
+// 11 planes.  Reloading it from global memory at those lags keeps a 12-plane window per
+// block live in L2, more than an XCD's L2 holds for its resident blocks.  Instead:
+//   stage 1 keeps the centre values of the q planes it reads from LDS (2-deep register
+//     ring), stage 2 reloads q at lag 5 (a window the L2 holds), and writes it into an
+//     LDS plane Q3 beside its output; stage 3 reads q from Q3 with its input plane, keeps
+//     it 2 planes in registers and forwards it through Q4 to stage 4 the same way.
 template <int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
 {
@@ -520,6 +547,8 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
     __shared__ __attribute__((aligned(16))) double P1[2][G2::AI];
     __shared__ __attribute__((aligned(16))) double P2[2][G3::AI];
     __shared__ __attribute__((aligned(16))) double P3[2][G4::AI];
+    __shared__ __attribute__((aligned(16))) double Q3[2][G3::AO + 2];  // q on stage 3's region
+    __shared__ __attribute__((aligned(16))) double Q4[2][G4::AO + 2];  // q on stage 4's region
 
     const int tid = threadIdx.x;
     const int ni = a.ni, nj = a.nj, nk = a.nk;
@@ -543,36 +572,47 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
         const int ax = 2 * (g % (G1::WI / 2)), by = g / (G1::WI / 2);
         off0[m] = (pmod(x0 - 8 + ax, ni) + pmod(y0 - 8 + by, nj) * isj) * 8;
     }
-    int q1[M1], q2[M2], q3[M3], q4[M4], o4[M4];
-    pair_offsets<TX, TY, 1, NT, M1>(q1, x0, y0, ni, nj, isj);
+    int q2[M2], o4[M4];
     pair_offsets<TX, TY, 2, NT, M2>(q2, x0, y0, ni, nj, isj);
-    pair_offsets<TX, TY, 3, NT, M3>(q3, x0, y0, ni, nj, isj);
-    pair_offsets<TX, TY, 4, NT, M4>(q4, x0, y0, ni, nj, isj);
 #pragma unroll
     for (int m = 0; m < M4; ++m) {
         const int g = min(tid + m * NT, G4::AO / 2 - 1);
         const int gx = x0 + 2 * (g % (TX / 2)), gy = y0 + g / (TX / 2);
         o4[m] = (pmod(gx, ni) + pmod(gy, nj) * osj) * 8;  // periodic image for partial tiles
     }
+    int s2[M2], w3[M2], r3[M3], w4[M3], r4[M4], n4[M4];
+    pair_qidx<TX, TY, 2, NT, M2>(s2, w3);
+    pair_qidx<TX, TY, 3, NT, M3>(r3, w4);
+    pair_qidx<TX, TY, 4, NT, M4>(r4, n4);
 
     double acc1[M1][2][4], acc2[M2][2][4], acc3[M3][2][4], acc4[M4][2][4];
     double d1[M1][2], d2v[M2][2], d3[M3][2], d4[M4][2];
-    d2 qv1[M1], qv2[M2], qv3[M3], qv4[M4];
+    d2 qv2[M2];
+    d2 c1[M1], c2[M2], c3[M3], c4[M4];                   // centres read this plane
+    d2 r1a[M1], r1b[M1], r3a[M3], r3b[M3], r4a[M4], r4b[M4];  // q one and two planes back
 #pragma unroll
-    for (int m = 0; m < M1; ++m)
+    for (int m = 0; m < M1; ++m) {
+        r1a[m] = r1b[m] = d2{0., 0.};
         for (int e = 0; e < 2; ++e) acc1[m][e][0] = acc1[m][e][1] = acc1[m][e][2] = acc1[m][e][3] = 0.;
+    }
 #pragma unroll
     for (int m = 0; m < M2; ++m)
         for (int e = 0; e < 2; ++e) acc2[m][e][0] = acc2[m][e][1] = acc2[m][e][2] = acc2[m][e][3] = 0.;
 #pragma unroll
-    for (int m = 0; m < M3; ++m)
+    for (int m = 0; m < M3; ++m) {
+        r3a[m] = r3b[m] = d2{0., 0.};
         for (int e = 0; e < 2; ++e) acc3[m][e][0] = acc3[m][e][1] = acc3[m][e][2] = acc3[m][e][3] = 0.;
+    }
 #pragma unroll
-    for (int m = 0; m < M4; ++m)
+    for (int m = 0; m < M4; ++m) {
+        r4a[m] = r4b[m] = d2{0., 0.};
         for (int e = 0; e < 2; ++e) acc4[m][e][0] = acc4[m][e][1] = acc4[m][e][2] = acc4[m][e][3] = 0.;
+    }
 
     // Same skewed schedule as k_rk4_fused: iteration t, stage 1: q t -> v1 t-2; stage 2:
-    // v1 t-3 -> v2 t-5; stage 3: v2 t-6 -> v3 t-8; stage 4: v3 t-9 -> q' t-11.
+    // v1 t-3 -> v2 t-5; stage 3: v2 t-6 -> v3 t-8; stage 4: v3 t-9 -> q' t-11.  Outputs
+    // of a stage's first iterations are incomplete and never consumed by a later stage's
+    // complete outputs (each stage starts when its first complete input plane exists).
     const int tq0 = z0 - 8, tq1 = zend + 8;
     const int tend = zend + 11;
     {
@@ -580,10 +620,7 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
 #pragma unroll
         for (int m = 0; m < M0; ++m)
             *(d2*)&P0[tq0 & 1][2 * min(tid + m * NT, A0P - 1)] = ld2(rin, off0[m], zo);
-        pair_qload<M1>(qv1, rin, wrapz(tq0 - 2, nk) * iskb, q1);
         pair_qload<M2>(qv2, rin, wrapz(z0 - 8, nk) * iskb, q2);
-        pair_qload<M3>(qv3, rin, wrapz(z0 - 6, nk) * iskb, q3);
-        pair_qload<M4>(qv4, rin, wrapz(z0 - 4, nk) * iskb, q4);
     }
     d2 pre[M0];
     if (tq0 + 1 < tq1) {
@@ -608,42 +645,55 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
         }
         __builtin_amdgcn_sched_barrier(0);
         if (t < tq1) {
-            pair_step<TX, TY, 1, NT, M1, R & 3>(P0[cur], w, acc1, d1);
+            // centres of plane t are q(t); r1b = q(t-2) is the update's q
+            pair_step<TX, TY, 1, NT, M1, R & 3>(P0[cur], w, acc1, d1, c1);
 #pragma unroll
-            for (int m = 0; m < M1; ++m)
+            for (int m = 0; m < M1; ++m) {
                 *(d2*)&P1[cur][2 * min(tid + m * NT, G1::AO / 2 - 1)] =
-                    d2{fmaT(w.f1, d1[m][0], qv1[m].x), fmaT(w.f1, d1[m][1], qv1[m].y)};
-            pair_qload<M1>(qv1, rin, wrapz(t - 1, nk) * iskb, q1);
+                    d2{fmaT(w.f1, d1[m][0], r1b[m].x), fmaT(w.f1, d1[m][1], r1b[m].y)};
+                r1b[m] = r1a[m];
+                r1a[m] = c1[m];
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
         if (t >= tq0 + 5 && t < tq1 + 1) {
-            pair_step<TX, TY, 2, NT, M2, (R + 1) & 3>(P1[prv], w, acc2, d2v);
+            pair_step<TX, TY, 2, NT, M2, (R + 1) & 3>(P1[prv], w, acc2, d2v, c2);
 #pragma unroll
-            for (int m = 0; m < M2; ++m)
+            for (int m = 0; m < M2; ++m) {
                 *(d2*)&P2[cur][2 * min(tid + m * NT, G2::AO / 2 - 1)] =
                     d2{fmaT(w.f2, d2v[m][0], qv2[m].x), fmaT(w.f2, d2v[m][1], qv2[m].y)};
+                *(d2*)&Q3[cur][w3[m]] = qv2[m];  // q(t-5) for stage 3
+            }
             pair_qload<M2>(qv2, rin, wrapz(t - 4, nk) * iskb, q2);
             __builtin_amdgcn_sched_barrier(0);
         }
         if (t >= tq0 + 10 && t < tq1 + 2) {
-            pair_step<TX, TY, 3, NT, M3, (R + 2) & 3>(P2[prv], w, acc3, d3);
+            pair_step<TX, TY, 3, NT, M3, (R + 2) & 3>(P2[prv], w, acc3, d3, c3);
 #pragma unroll
-            for (int m = 0; m < M3; ++m)
+            for (int m = 0; m < M3; ++m) {
+                const d2 qn = *(const d2*)&Q3[prv][r3[m]];  // q(t-6), written by stage 2 last plane
                 *(d2*)&P3[cur][2 * min(tid + m * NT, G3::AO / 2 - 1)] =
-                    d2{fmaT(w.f3, d3[m][0], qv3[m].x), fmaT(w.f3, d3[m][1], qv3[m].y)};
-            pair_qload<M3>(qv3, rin, wrapz(t - 7, nk) * iskb, q3);
+                    d2{fmaT(w.f3, d3[m][0], r3b[m].x), fmaT(w.f3, d3[m][1], r3b[m].y)};
+                *(d2*)&Q4[cur][w4[m]] = r3b[m];  // q(t-8) for stage 4
+                r3b[m] = r3a[m];
+                r3a[m] = qn;
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
         if (t >= tq0 + 15) {
-            pair_step<TX, TY, 4, NT, M4, (R + 3) & 3>(P3[prv], w, acc4, d4);
+            pair_step<TX, TY, 4, NT, M4, (R + 3) & 3>(P3[prv], w, acc4, d4, c4);
             const int zp = t - 11;
             if (zp >= z0 && zp < zend) {
                 const int zo = zp * oskb;
 #pragma unroll
                 for (int m = 0; m < M4; ++m)
-                    st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], qv4[m].x), fmaT(w.f4, d4[m][1], qv4[m].y)});
+                    st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], r4b[m].x), fmaT(w.f4, d4[m][1], r4b[m].y)});
             }
-            pair_qload<M4>(qv4, rin, wrapz(t - 10, nk) * iskb, q4);
+#pragma unroll
+            for (int m = 0; m < M4; ++m) {
+                r4b[m] = r4a[m];
+                r4a[m] = *(const d2*)&Q4[prv][r4[m]];  // q(t-9)
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
@@ -749,10 +799,7 @@ int launch_pair(hipStream_t s, const FusedStep& st, const FusedTile& tile, View 
         SP_HIP(hipGetLastError());                                                       \
         return SP_OK;                                                                    \
     }
-    SP_PAIR_CASE(32, 32, 1024)
-    SP_PAIR_CASE(32, 32, 768)
     SP_PAIR_CASE(32, 32, 512)
-    SP_PAIR_CASE(32, 24, 768)
     SP_PAIR_CASE(32, 16, 512)
     SP_PAIR_CASE(16, 16, 256)
 #undef SP_PAIR_CASE
@@ -772,8 +819,7 @@ bool pair_ok(const FusedStep& st, View in, View out)
 bool fused_tile_supported(const FusedTile& t, int /*compute_f64*/)
 {
     static const int scalar[][3] = {{32, 32, 1024}, {32, 24, 1024}, {24, 24, 1024}, {32, 16, 512}, {16, 16, 256}};
-    static const int pair[][3] = {{32, 32, 1024}, {32, 32, 768}, {32, 32, 512},
-                                  {32, 24, 768},  {32, 16, 512}, {16, 16, 256}};
+    static const int pair[][3] = {{32, 32, 512}, {32, 16, 512}, {16, 16, 256}};
     if (t.variant == SP_KERNEL_PAIR) {
         for (const auto& sh : pair)
             if (t.tx == sh[0] && t.ty == sh[1] && t.nt == sh[2]) return true;

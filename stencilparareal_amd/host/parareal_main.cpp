@@ -15,6 +15,7 @@
 
 #include "sp/Convection.hpp"
 #include "sp/ConvectionField.hpp"
+#include "sp/MatFile.hpp"
 #include "sp/Parareal.hpp"
 #include "sp/RuntimeConfiguration.hpp"
 
@@ -150,6 +151,10 @@ int main(int argc, char** argv)
                       << "Maximal speedup: " << maxSpeedup << "\n"
                       << "Error at end: " << e << "\n"
                       << std::endl;
+
+            MatFile matfile("result.mat");  // main.cpp:251-253
+            matfile.addField("q", q);
+            matfile.addField("qreference", qreference);
         } else if (conf.mode() == ModeSerial) {
             fillQ(qinitial, conf, 0.);
             qinitial.SynchronizeDeviceStorage();
