@@ -40,6 +40,13 @@ public:
 
     // extensions: the reference-order kernel path and the Advection2 branch of DoEuler
     void setStaged(bool staged) { check(sp_prop_set_path(p_, staged ? SP_PATH_STAGED : SP_PATH_FUSED)); }
+    // times the fused kernel's tiles for `slices` FP64 slices and keeps the fastest
+    double autotune(int slices = 1)
+    {
+        double us = 0.;
+        check(sp_prop_autotune(p_, slices, SP_F64, SP_F64, &us));
+        return us;
+    }
     void setUpwind(bool upwind) { check(sp_prop_set_upwind(p_, upwind ? 1 : 0)); }
     sp_prop* handle() { return p_; }
 

@@ -23,6 +23,9 @@ struct PararealOptions {
     int coarse_substeps = 1;
     double tol = 0.;
     bool use_graphs = false;
+    bool autotune = false;
+    int correction = -1;  // -1: difference form for f32 / bf16 coarse storage, else the
+                          // reference's; 0: the reference's; 1: difference form
 };
 
 template <typename PropagatorT, typename FieldType>
@@ -43,6 +46,8 @@ public:
         c.coarse_substeps = opt.coarse_substeps;
         c.tol = opt.tol;
         c.async_schedule = conf.async() ? 1 : 0;
+        c.autotune = opt.autotune ? 1 : 0;
+        c.delta_correction = opt.correction < 0 ? (opt.coarse_dtype != SP_F64 ? 1 : 0) : opt.correction;
         c.use_graphs = opt.use_graphs ? 1 : 0;
         check(sp_parareal_create(propagator.handle(), &c, &pr_));
     }

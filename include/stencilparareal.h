@@ -122,6 +122,14 @@ int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int 
  * kernel instead (no counterpart in the reference; reports what is launched). */
 int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* tx, int* ty, int* nthreads,
                      int* kchunk);
+/* Autotuner (no counterpart in the reference, whose CUDA block size is fixed at 32x4):
+ * times every compiled fused-kernel tile shape and k chunk for `nslices` slices of this
+ * propagator's size with the given storage types, and keeps the fastest for later
+ * sp_do_rk4* calls with that slice count and compute type (FP64 when both types are
+ * SP_F64, else FP32), unless a tile is set with sp_prop_set_tile.  Synchronous; allocates
+ * two scratch batches for the duration; refused while the stream is being captured.
+ * *best_us (nullable) receives the fastest step time in microseconds. */
+int sp_prop_autotune(sp_prop* p, int nslices, int in_dtype, int out_dtype, double* best_us);
 
 /* DoRK4: `steps` RK4 steps of size dt from tStart; step i runs at tStart + i*dt.
  * in == out is allowed.  steps == 0 leaves out untouched.  in and out may have
@@ -186,6 +194,11 @@ typedef struct sp_parareal_cfg {
     double tol;          /* stop once the relative L2 increment <= tol; 0: run kmax */
     int async_schedule;  /* 1: fine-step times of the reference's async mode */
     int use_graphs;      /* 1: capture each iteration (fine sweep, coarse chain, updates) as one hipGraph */
+    int autotune;        /* 1: autotune the fused-kernel tiles of F and G at create (sp_prop_autotune) */
+    int delta_correction; /* 0: the reference's update G(U_n^{k+1}) + F(U_n^k) - G(U_n^k)
+                             (Parareal.h:19-35); 1: F(U_n^k) + G(U_n^{k+1} - U_n^k), equal for
+                             this linear G, whose rounding then shrinks with the increment --
+                             the form for SP_F32 / SP_BF16 coarse storage */
 } sp_parareal_cfg;
 
 #define SP_MAX_ITERS 64

@@ -32,7 +32,7 @@ EXPORTS = [
     "sp_version", "sp_last_error", "sp_ctx_create", "sp_ctx_destroy", "sp_ctx_set_stream",
     "sp_ctx_get_stream", "sp_ctx_sync", "sp_malloc", "sp_free", "sp_memcpy_h2d", "sp_memcpy_d2h",
     "sp_layout_compact", "sp_layout_stella_cuda", "sp_layout_storage_elems", "sp_prop_create",
-    "sp_prop_destroy", "sp_prop_set_upwind", "sp_prop_set_path", "sp_prop_set_tile", "sp_prop_get_tile", "sp_do_rk4",
+    "sp_prop_destroy", "sp_prop_set_upwind", "sp_prop_set_path", "sp_prop_set_tile", "sp_prop_get_tile", "sp_prop_autotune", "sp_do_rk4",
     "sp_do_euler", "sp_do_rk4_timestep", "sp_do_euler_timestep", "sp_do_rk4_ex", "sp_do_rk4_batched",
     "sp_periodic_fill",
     "sp_parareal_update", "sp_norms", "sp_fill_exact", "sp_fill_gaussians", "sp_parareal_create",
@@ -57,7 +57,8 @@ class PararealCfg(C.Structure):
     _fields_ = [("slices", C.c_int), ("kmax", C.c_int), ("end_time", C.c_double),
                 ("steps_fine", C.c_int), ("steps_coarse", C.c_int), ("coarse_kind", C.c_int),
                 ("coarse_dtype", C.c_int), ("coarse_substeps", C.c_int), ("tol", C.c_double),
-                ("async_schedule", C.c_int), ("use_graphs", C.c_int)]
+                ("async_schedule", C.c_int), ("use_graphs", C.c_int), ("autotune", C.c_int),
+                ("delta_correction", C.c_int)]
 
 
 class PararealReport(C.Structure):
@@ -93,6 +94,7 @@ _SIGS = {
     "sp_prop_set_path": (C.c_int, [_vp, C.c_int]),
     "sp_prop_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "sp_prop_get_tile": (C.c_int, [_vp, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 5),
+    "sp_prop_autotune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "sp_do_rk4": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
     "sp_do_euler": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
     "sp_do_rk4_timestep": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double]),

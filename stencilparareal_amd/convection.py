@@ -53,6 +53,14 @@ class Convection:
         return {"variant": {L.SP_KERNEL_SCALAR: "scalar", L.SP_KERNEL_PAIR: "pair"}.get(var, var), "tx": tx,
                 "ty": ty, "nthreads": nt, "kchunk": kc}
 
+    def autotune(self, nslices: int = 1, in_dtype: str = "f64", out_dtype: str = "f64") -> float:
+        """Times the fused kernel's tile shapes and k chunks for `nslices` slices and keeps
+        the fastest (sp_prop_autotune); returns its step time in microseconds."""
+        self.ctx.sync_stream()
+        us = C.c_double()
+        L.call("sp_prop_autotune", self.h, nslices, L.DTYPE_CODES[in_dtype], L.DTYPE_CODES[out_dtype], C.byref(us))
+        return us.value
+
     def computenu(self, t: float) -> float:
         import math
         return self.nu0 * (1. + .5 * math.sin(self.nufreq * t))

@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "sp_internal.hpp"
@@ -88,6 +90,7 @@ struct sp_prop {
     Buf qI, k1, k2, k3, k4;         // staged path temporaries (f64, compact)
     Buf nu_tab;                     // per-step nu table
     std::vector<double> nu_host;
+    std::map<std::pair<int, int>, FusedTile> tuned;  // (slices, FP64 compute) -> autotuned tile
 };
 
 namespace {
@@ -127,6 +130,8 @@ int check_prop_layout(const sp_prop* p, const sp_layout* l, const void* ptr, con
 FusedTile pick_tile(const sp_prop* p, int nslices, int f64)
 {
     FusedTile t = default_tile(p->ni, p->nj, p->nk, nslices, f64);
+    const auto hit = p->tuned.find({nslices, f64 ? 1 : 0});
+    if (p->tile.tx == 0 && hit != p->tuned.end()) return hit->second;
     if (p->tile.tx > 0) {
         t.tx = p->tile.tx;
         t.ty = p->tile.ty;
@@ -152,6 +157,58 @@ FusedStep fused_step(const sp_prop* p, int nslices, double dt, const void* coef_
     st.coef_dev = coef_dev;
     st.nu = nu;
     return st;
+}
+
+// Autotuner: times every candidate tile / k chunk of the fused kernel for `nslices` slices
+// (views in -> out, slice stride set) and records the fastest for pick_tile.  Synchronous;
+// refuses to run while the stream is being captured.
+int tune_fused(sp_prop* p, int nslices, int idt, int odt, View in, View out, double* best_us)
+{
+    hipStream_t s = p->ctx->stream;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SP_HIP(hipStreamIsCapturing(s, &cs));
+    SP_CHECK(cs == hipStreamCaptureStatusNone, SP_ERR_INVALID, "autotune: stream is being captured");
+    const int f64 = idt == SP_F64 && odt == SP_F64;
+    std::vector<FusedTile> cand;
+    fused_candidates(p->ni, p->nj, p->nk, nslices, f64, cand);
+    hipEvent_t e0, e1;
+    SP_HIP(hipEventCreate(&e0));
+    SP_HIP(hipEventCreate(&e1));
+    const FusedStep st = fused_step(p, nslices, 1e-6, nullptr, p->nu0);
+    double best = 1e30;
+    FusedTile best_t = cand.front();
+    int r = SP_OK;
+    for (const FusedTile& t : cand) {
+        // one untimed launch, then the mean of up to 3 (one when a launch is long)
+        float ms = 0.f;
+        r = launch_rk4_fused(s, st, t, idt, in, odt, out);
+        if (r == SP_OK && hipEventRecord(e0, s) != hipSuccess) r = fail(SP_ERR_HIP, "autotune: event");
+        if (r == SP_OK) r = launch_rk4_fused(s, st, t, idt, in, odt, out);
+        if (r == SP_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            r = fail(SP_ERR_HIP, "autotune: timing");
+        if (r != SP_OK) break;
+        double us = ms * 1e3;
+        if (ms < 20.f) {
+            if (hipEventRecord(e0, s) != hipSuccess) r = fail(SP_ERR_HIP, "autotune: event");
+            for (int i = 0; i < 3 && r == SP_OK; ++i) r = launch_rk4_fused(s, st, t, idt, in, odt, out);
+            if (r == SP_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                r = fail(SP_ERR_HIP, "autotune: timing");
+            if (r != SP_OK) break;
+            us = ms * 1e3 / 3.;
+        }
+        if (us < best) {
+            best = us;
+            best_t = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SP_TRY(r);
+    p->tuned[{nslices, f64}] = best_t;
+    if (best_us) *best_us = best;
+    return SP_OK;
 }
 
 // RK4 steps with the fused kernel on one field; in/out views and dtypes given.
@@ -449,6 +506,31 @@ int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* 
     return SP_OK;
 }
 
+int sp_prop_autotune(sp_prop* p, int nslices, int in_dtype, int out_dtype, double* best_us)
+{
+    SP_CHECK(p, SP_ERR_INVALID, "sp_prop_autotune: null propagator");
+    SP_CHECK(nslices >= 1, SP_ERR_INVALID, "sp_prop_autotune: nslices < 1");
+    SP_CHECK(in_dtype >= SP_F64 && in_dtype <= SP_BF16 && out_dtype >= SP_F64 && out_dtype <= SP_BF16,
+             SP_ERR_INVALID, "sp_prop_autotune: dtype");
+    SP_CHECK(p->nk >= kFusedMinK, SP_ERR_UNSUPPORTED, "sp_prop_autotune: ksize below the fused kernel's minimum");
+    SP_HIP(hipSetDevice(p->ctx->device));
+    const size_t n = (size_t)p->ni * p->nj * p->nk;
+    Buf a, b;
+    int r = a.ensure(n * nslices * dtype_size(in_dtype));
+    if (r == SP_OK) r = b.ensure(n * nslices * dtype_size(out_dtype));
+    if (r == SP_OK && hipMemsetAsync(a.p, 0, a.cap, p->ctx->stream) != hipSuccess)
+        r = fail(SP_ERR_HIP, "sp_prop_autotune: clear");
+    if (r == SP_OK) {
+        View vi = cview(a.p, p->ni, p->nj), vo = cview(b.p, p->ni, p->nj);
+        vi.slice = vo.slice = (long long)n;
+        r = tune_fused(p, nslices, in_dtype, out_dtype, vi, vo, best_us);
+    }
+    (void)hipStreamSynchronize(p->ctx->stream);
+    a.release();
+    b.release();
+    return r;
+}
+
 static int rk4_common(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
                       double tStart, double dt, int steps, bool f64_only, const char* name)
 {
@@ -671,6 +753,7 @@ struct sp_parareal {
     int gdt = SP_F64;
     size_t n = 0;                 // points per slice
     Buf U, F, FS, Gold, Gw0, Gw1, Q, coefF, coefG, part;
+    Buf D;  // difference form: U_n^{k+1} - U_n^k of the chain's current slice
     std::vector<double> nuF_host, nuG_host;
     bool nu_const = true;
     bool begun = false;
@@ -808,6 +891,32 @@ int coarse_chain(sp_parareal* pr)
     return SP_OK;
 }
 
+// The chain in difference form (delta_correction): U_{r+1} = F(U_r^k) + G(U_r^{k+1} - U_r^k),
+// equal to the reference's G(U_r^{k+1}) + F(U_r^k) - G(U_r^k) for the linear G.  D carries
+// the difference from one slice to the next; slice 0's is zero (its start state is u0).
+int coarse_chain_delta(sp_parareal* pr)
+{
+    sp_prop* p = pr->p;
+    hipStream_t s = p->ctx->stream;
+    const View D = cview(pr->D.p, p->ni, p->nj);
+    for (int r = 0; r < pr->S; ++r) {
+        View gd = cview(nullptr, p->ni, p->nj);
+        if (r > 0) {
+            const View A = cview(pr->Gw0.p, p->ni, p->nj), B = cview(pr->Gw1.p, p->ni, p->nj);
+            SP_TRY(coarse_run(pr, 1, r, SP_F64, D, A, B, pr->coefG.p, -1, pr->tsG));
+            gd = A;
+        }
+        const View fine = cview((char*)pr->F.p + (size_t)r * pr->n * 8, p->ni, p->nj);
+        const bool last = r == pr->S - 1;
+        const View dst = last ? cview(pr->Q.p, p->ni, p->nj)
+                              : cview((char*)pr->U.p + (size_t)(r + 1) * pr->n * 8, p->ni, p->nj);
+        SP_TRY(launch_correction_delta(s, p->ni, p->nj, p->nk, pr->gdt, gd, fine, dst,
+                                       last ? cview(nullptr, p->ni, p->nj) : D,
+                                       (double*)pr->part.p + (size_t)r * kReduceBlocks * 2, kReduceBlocks));
+    }
+    return SP_OK;
+}
+
 double now()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -832,6 +941,8 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
              "sp_parareal_create: Euler coarse propagator is SP_F64 only");
     SP_CHECK(cfg->coarse_substeps >= 1, SP_ERR_INVALID, "sp_parareal_create: coarse_substeps < 1");
     SP_CHECK(cfg->tol >= 0, SP_ERR_INVALID, "sp_parareal_create: tol < 0");
+    SP_CHECK(cfg->delta_correction == 0 || cfg->delta_correction == 1, SP_ERR_INVALID,
+             "sp_parareal_create: delta_correction must be 0 or 1");
     SP_HIP(hipSetDevice(p->ctx->device));
     sp_parareal* pr = new sp_parareal;
     pr->p = p;
@@ -858,7 +969,9 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
     chk(pr->U.ensure(nb * 8));
     chk(pr->F.ensure(nb * 8));
     chk(pr->FS.ensure(nb * 8));
-    chk(pr->Gold.ensure(nb * dtype_size(pr->gdt)));
+    // the difference form keeps no coarse values: one slice of Gold serves as scratch
+    chk(pr->Gold.ensure((cfg->delta_correction ? pr->n : nb) * dtype_size(pr->gdt)));
+    if (cfg->delta_correction) chk(pr->D.ensure(pr->n * 8));
     chk(pr->Gw0.ensure(pr->n * dtype_size(pr->gdt)));
     chk(pr->Gw1.ensure(pr->n * dtype_size(pr->gdt)));
     chk(pr->Q.ensure(pr->n * 8));
@@ -902,6 +1015,29 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
             return r;
         }
     }
+    if (cfg->autotune && p->path == SP_PATH_FUSED && p->nk >= kFusedMinK && p->tile.tx == 0) {
+        // tune on the Parareal's own (cleared) slice arrays, before any capture
+        hipStream_t s = p->ctx->stream;
+        if (hipMemsetAsync(pr->U.p, 0, pr->U.cap, s) != hipSuccess ||
+            hipMemsetAsync(pr->F.p, 0, pr->F.cap, s) != hipSuccess ||
+            hipMemsetAsync(pr->Gw0.p, 0, pr->Gw0.cap, s) != hipSuccess)
+            r = fail(SP_ERR_HIP, "sp_parareal_create: clear");
+        View u = cview(pr->U.p, p->ni, p->nj), f = cview(pr->FS.p, p->ni, p->nj);
+        u.slice = f.slice = (long long)pr->n;
+        if (r == SP_OK) r = tune_fused(p, pr->S, SP_F64, SP_F64, u, f, nullptr);
+        if (r == SP_OK && cfg->coarse_kind == SP_COARSE_RK4) {
+            View gi = cview(pr->F.p, p->ni, p->nj), go = cview(pr->Gold.p, p->ni, p->nj);
+            gi.slice = go.slice = (long long)pr->n;
+            if (pr->S > 1 && !cfg->delta_correction) r = tune_fused(p, pr->S, pr->gdt, pr->gdt, gi, go, nullptr);
+            if (r == SP_OK)
+                r = tune_fused(p, 1, pr->gdt, pr->gdt, cview(pr->Gw0.p, p->ni, p->nj), cview(pr->Gw1.p, p->ni, p->nj),
+                               nullptr);
+        }
+        if (r != SP_OK) {
+            sp_parareal_destroy(pr);
+            return r;
+        }
+    }
     pr->tileF = pick_tile(p, pr->S, 1);
     pr->tileG1 = pick_tile(p, 1, pr->gdt == SP_F64);
     pr->tileGS = pick_tile(p, pr->S, pr->gdt == SP_F64);
@@ -916,7 +1052,8 @@ int sp_parareal_destroy(sp_parareal* pr)
     (void)hipStreamSynchronize(pr->p->ctx->stream);
     if (pr->gexec) (void)hipGraphExecDestroy(pr->gexec);
     if (pr->graph) (void)hipGraphDestroy(pr->graph);
-    for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->coefF, &pr->coefG, &pr->part})
+    for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->coefF, &pr->coefG, &pr->part,
+                   &pr->D})
         b->release();
     sp_prop* p = pr->p;
     delete pr;
@@ -954,7 +1091,8 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
         }
     }
     // Gold_r = G(U_r) from tStart_r (Parareal.h:114), all slices in one batch; FS is scratch.
-    {
+    // (The difference form needs no coarse values.)
+    if (!pr->cfg.delta_correction) {
         View src = batch_view(pr->U.p, pr, SP_F64);
         View dst = batch_view(pr->Gold.p, pr, pr->gdt);
         View scr = batch_view(pr->FS.p, pr, pr->gdt);
@@ -982,9 +1120,16 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
         pr->u0_norm2 = n2;
     }
     // The coarse prediction of the final state is the last slice's coarse value.
-    SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt,
-                          cview((char*)pr->Gold.p + (size_t)(pr->S - 1) * pr->n * gs, p->ni, p->nj), SP_F64,
-                          cview(pr->Q.p, p->ni, p->nj)));
+    if (pr->cfg.delta_correction) {
+        const View A = cview(pr->Gw0.p, p->ni, p->nj), B = cview(pr->Gw1.p, p->ni, p->nj);
+        SP_TRY(coarse_run(pr, 1, pr->S - 1, SP_F64, cview((char*)pr->U.p + (size_t)(pr->S - 1) * pr->n * 8, p->ni, p->nj),
+                          A, B, pr->coefG.p, -1, pr->tsG));
+        SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt, A, SP_F64, cview(pr->Q.p, p->ni, p->nj)));
+    } else {
+        SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt,
+                              cview((char*)pr->Gold.p + (size_t)(pr->S - 1) * pr->n * gs, p->ni, p->nj), SP_F64,
+                              cview(pr->Q.p, p->ni, p->nj)));
+    }
     SP_HIP(hipStreamSynchronize(s));
     pr->begun = true;
     pr->iters_done = 0;
@@ -1008,7 +1153,7 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
             p->ctx->stream = s;
             SP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int r = fine_sweep(pr);
-            if (r == SP_OK) r = coarse_chain(pr);
+            if (r == SP_OK) r = (pr->cfg.delta_correction ? coarse_chain_delta(pr) : coarse_chain(pr));
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(s, &g);
             p->ctx->stream = user;
@@ -1028,7 +1173,7 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
         SP_TRY(fine_sweep(pr));
         SP_HIP(hipStreamSynchronize(s));
         const double b = now();
-        SP_TRY(coarse_chain(pr));
+        SP_TRY((pr->cfg.delta_correction ? coarse_chain_delta(pr) : coarse_chain(pr)));
         SP_HIP(hipStreamSynchronize(s));
         if (tfine) *tfine = b - a;
         if (tcoarse) *tcoarse = now() - b;

@@ -32,6 +32,8 @@
 //    per lattice update) plus the x-y halo overlap of neighbouring tiles.
 //  * The plane for iteration t+1 is loaded into registers while iteration t computes.
 
+#include <algorithm>
+
 #include "sp_internal.hpp"
 
 #ifndef SP_SERIAL_SLOTS
@@ -420,28 +422,55 @@ __device__ __forceinline__ void st2(__amdgpu_buffer_rsrc_t r, int vo, int so, d2
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, vo, so, 0);
 }
 
-// Pair g of a stage region -> element offset of its first column in the region (row-major,
-// WO columns per row, WO even): simply 2g.  Its LDS centre in the input plane is
-// (by + 2) * WI + ax + 2 with ax = 2 (g % (WO/2)), by = g / (WO/2).
-template <int TX, int TY, int S>
-__device__ __forceinline__ int pair_centre(int g)
-{
+// Lane-slot layout of stage S's output pairs (WP pairs x H rows).  Slot g maps to pair
+// (ax / 2, by), row-major with VW slots per row.  With VW equal to the input plane's row
+// stride in pairs (WI / 2), consecutive slots read consecutive 16-byte LDS chunks across
+// row ends too, so no 16-lane group of a ds_read_b128 touches a bank twice; the slots past
+// a row's last pair repeat it (same address: a broadcast, and the same value written).
+// That layout is used whenever it needs no more passes of NT threads than the dense one
+// (VW = WP), whose groups straddling a row end hit two banks twice, unless rows are whole
+// 16-lane groups already.
+template <int TX, int TY, int S, int NT>
+struct PairMap {
     using G = Geo<TX, TY, S>;
-    constexpr int WP = G::WO / 2;
-    return (g / WP + 2) * G::WI + 2 * (g % WP) + 2;
-}
+    static constexpr int WP = G::WO / 2, H = G::HOo;
+    static constexpr int DENSE = WP * H, PADDED = (G::WI / 2) * H;
+    static constexpr bool PAD = WP % 16 != 0 && (PADDED + NT - 1) / NT == (DENSE + NT - 1) / NT;
+    static constexpr int VW = PAD ? G::WI / 2 : WP;
+    static constexpr int SLOTS = VW * H;
+    static constexpr int M = (SLOTS + NT - 1) / NT;  // passes
+    // pair of slot g: column ax (doubles) and row by of the output region
+    __device__ static void at(int g, int& ax, int& by)
+    {
+        g = min(g, SLOTS - 1);
+        by = g / VW;
+        ax = 2 * min(g % VW, WP - 1);
+    }
+    // its centre in the input plane (row stride WI), and its place in the output plane
+    __device__ static int centre(int g)
+    {
+        int ax, by;
+        at(g, ax, by);
+        return (by + 2) * G::WI + ax + 2;
+    }
+    __device__ static int out(int g)
+    {
+        int ax, by;
+        at(g, ax, by);
+        return by * G::WO + ax;
+    }
+};
 
 template <int TX, int TY, int S, int NT, int M, int R>
 __device__ __forceinline__ void pair_step(const double* __restrict__ P, const StarW<double>& w, double (&acc)[M][2][4],
                                           double (&done)[M][2], d2 (&cen)[M])
 {
     using G = Geo<TX, TY, S>;
-    constexpr int WI = G::WI, AP = G::AO / 2;
+    constexpr int WI = G::WI;
     constexpr int sm2 = (R + 2) & 3, sm1 = (R + 3) & 3, s0 = R & 3, sp1 = (R + 1) & 3;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const int g = min((int)threadIdx.x + m * NT, AP - 1);
-        const double* c = P + pair_centre<TX, TY, S>(g);
+        const double* c = P + PairMap<TX, TY, S, NT>::centre((int)threadIdx.x + m * NT);
         // x group first (centres and x neighbours), then the y group: at most 4 pairs of
         // loaded values are live at a time
         const d2 C = *(const d2*)c, Lx = *(const d2*)(c - 2), Rx = *(const d2*)(c + 2);
@@ -487,11 +516,10 @@ template <int TX, int TY, int S, int NT, int M>
 __device__ __forceinline__ void pair_offsets(int (&q)[M], int x0, int y0, int ni, int nj, int sj)
 {
     using G = Geo<TX, TY, S>;
-    constexpr int WP = G::WO / 2, AP = G::AO / 2;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const int g = min((int)threadIdx.x + m * NT, AP - 1);
-        const int ax = 2 * (g % WP), by = g / WP;
+        int ax, by;
+        PairMap<TX, TY, S, NT>::at((int)threadIdx.x + m * NT, ax, by);
         q[m] = (pmod(x0 - G::HO + ax, ni) + pmod(y0 - G::HO + by, nj) * sj) * 8;
     }
 }
@@ -510,11 +538,11 @@ template <int TX, int TY, int S, int NT, int M>
 __device__ __forceinline__ void pair_qidx(int (&same)[M], int (&next)[M])
 {
     using G = Geo<TX, TY, S>;
-    constexpr int WP = G::WO / 2, AP = G::AO / 2, WN = G::WO - 4, AN = WN * (G::HOo - 4);
+    constexpr int WN = G::WO - 4, AN = WN * (G::HOo - 4);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const int g = min((int)threadIdx.x + m * NT, AP - 1);
-        const int ax = 2 * (g % WP), by = g / WP;
+        int ax, by;
+        PairMap<TX, TY, S, NT>::at((int)threadIdx.x + m * NT, ax, by);
         same[m] = by * G::WO + ax;
         const bool in = ax >= 2 && ax < G::WO - 2 && by >= 2 && by < G::HOo - 2;
         next[m] = in ? (by - 2) * WN + ax - 2 : AN;  // AN: a dump pair past the plane
@@ -538,10 +566,11 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
     using G4 = Geo<TX, TY, 4>;
     constexpr int A0P = G1::AI / 2;  // pairs of the q plane incl. its 8-cell halo
     constexpr int M0 = (A0P + NT - 1) / NT;
-    constexpr int M1 = (G1::AO / 2 + NT - 1) / NT;
-    constexpr int M2 = (G2::AO / 2 + NT - 1) / NT;
-    constexpr int M3 = (G3::AO / 2 + NT - 1) / NT;
-    constexpr int M4 = (G4::AO / 2 + NT - 1) / NT;
+    using PM1 = PairMap<TX, TY, 1, NT>;
+    using PM2 = PairMap<TX, TY, 2, NT>;
+    using PM3 = PairMap<TX, TY, 3, NT>;
+    using PM4 = PairMap<TX, TY, 4, NT>;
+    constexpr int M1 = PM1::M, M2 = PM2::M, M3 = PM3::M, M4 = PM4::M;
 
     __shared__ __attribute__((aligned(16))) double P0[2][G1::AI];
     __shared__ __attribute__((aligned(16))) double P1[2][G2::AI];
@@ -576,9 +605,9 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
     pair_offsets<TX, TY, 2, NT, M2>(q2, x0, y0, ni, nj, isj);
 #pragma unroll
     for (int m = 0; m < M4; ++m) {
-        const int g = min(tid + m * NT, G4::AO / 2 - 1);
-        const int gx = x0 + 2 * (g % (TX / 2)), gy = y0 + g / (TX / 2);
-        o4[m] = (pmod(gx, ni) + pmod(gy, nj) * osj) * 8;  // periodic image for partial tiles
+        int ax, by;
+        PM4::at(tid + m * NT, ax, by);
+        o4[m] = (pmod(x0 + ax, ni) + pmod(y0 + by, nj) * osj) * 8;  // periodic image for partial tiles
     }
     int s2[M2], w3[M2], r3[M3], w4[M3], r4[M4], n4[M4];
     pair_qidx<TX, TY, 2, NT, M2>(s2, w3);
@@ -649,7 +678,7 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
             pair_step<TX, TY, 1, NT, M1, R & 3>(P0[cur], w, acc1, d1, c1);
 #pragma unroll
             for (int m = 0; m < M1; ++m) {
-                *(d2*)&P1[cur][2 * min(tid + m * NT, G1::AO / 2 - 1)] =
+                *(d2*)&P1[cur][PM1::out(tid + m * NT)] =
                     d2{fmaT(w.f1, d1[m][0], r1b[m].x), fmaT(w.f1, d1[m][1], r1b[m].y)};
                 r1b[m] = r1a[m];
                 r1a[m] = c1[m];
@@ -660,7 +689,7 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
             pair_step<TX, TY, 2, NT, M2, (R + 1) & 3>(P1[prv], w, acc2, d2v, c2);
 #pragma unroll
             for (int m = 0; m < M2; ++m) {
-                *(d2*)&P2[cur][2 * min(tid + m * NT, G2::AO / 2 - 1)] =
+                *(d2*)&P2[cur][PM2::out(tid + m * NT)] =
                     d2{fmaT(w.f2, d2v[m][0], qv2[m].x), fmaT(w.f2, d2v[m][1], qv2[m].y)};
                 *(d2*)&Q3[cur][w3[m]] = qv2[m];  // q(t-5) for stage 3
             }
@@ -672,7 +701,7 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
 #pragma unroll
             for (int m = 0; m < M3; ++m) {
                 const d2 qn = *(const d2*)&Q3[prv][r3[m]];  // q(t-6), written by stage 2 last plane
-                *(d2*)&P3[cur][2 * min(tid + m * NT, G3::AO / 2 - 1)] =
+                *(d2*)&P3[cur][PM3::out(tid + m * NT)] =
                     d2{fmaT(w.f3, d3[m][0], r3b[m].x), fmaT(w.f3, d3[m][1], r3b[m].y)};
                 *(d2*)&Q4[cur][w4[m]] = r3b[m];  // q(t-8) for stage 4
                 r3b[m] = r3a[m];
@@ -851,6 +880,31 @@ FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64)
     if (kc < 16) kc = 16;
     t.kchunk = kc >= nk ? 0 : kc;
     return t;
+}
+
+void fused_candidates(int ni, int nj, int nk, int nslices, int compute_f64, std::vector<FusedTile>& out)
+{
+    static const int scalar[][3] = {{32, 32, 1024}, {32, 24, 1024}, {24, 24, 1024}, {32, 16, 512}, {16, 16, 256}};
+    static const int pair[][3] = {{32, 32, 512}, {32, 16, 512}, {16, 16, 256}};
+    std::vector<FusedTile> shapes;
+    if (compute_f64 && ni % 2 == 0)
+        for (const auto& sh : pair) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_PAIR});
+    for (const auto& sh : scalar) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_SCALAR});
+    out.clear();
+    for (const FusedTile& t : shapes) {
+        const long long blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
+        // k chunks: the whole extent, and shorter ones (>= 16 planes) only while the tiles
+        // alone give fewer than 4 blocks per CU
+        std::vector<int> kcs{0};
+        if (blocks < 1024)
+            for (int c : {nk / 2, nk / 4, nk / 8, 64, 32, 16})
+                if (c >= 16 && c < nk && std::find(kcs.begin(), kcs.end(), c) == kcs.end()) kcs.push_back(c);
+        for (int c : kcs) {
+            FusedTile x = t;
+            x.kchunk = c;
+            out.push_back(x);
+        }
+    }
 }
 
 int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile_in, int in_dtype, View in,

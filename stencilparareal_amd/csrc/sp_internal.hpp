@@ -153,6 +153,8 @@ int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile, 
                      int out_dtype, View out);
 bool fused_tile_supported(const FusedTile& t, int compute_f64);
 FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64);
+// Tile shapes and k chunks the autotuner times for this size and batch.
+void fused_candidates(int ni, int nj, int nk, int nslices, int compute_f64, std::vector<FusedTile>& out);
 
 // Misc kernels.
 int launch_periodic_fill(hipStream_t s, const sp_layout& l, void* base);
@@ -162,6 +164,8 @@ int launch_convert(hipStream_t s, int ni, int nj, int nk, int nslices, int in_dt
 // Parareal correction: q = cnew + fine - cold (f64), cold <- cnew, increment partials.
 int launch_correction(hipStream_t s, int ni, int nj, int nk, int g_dtype, View cnew, View cold, View fine,
                       View dst, double* partials, int nblocks);
+int launch_correction_delta(hipStream_t s, int ni, int nj, int nk, int g_dtype, View gd, View fine, View dst,
+                            View dnext, double* partials, int nblocks);
 int launch_norms(hipStream_t s, int ni, int nj, int nk, View a, View b, double* partials, int nblocks);
 int launch_fill_exact(hipStream_t s, const sp_layout& l, void* base, double nu0, double nufreq, double cx,
                       double cy, double cz, double t);

@@ -120,6 +120,38 @@ __global__ void __launch_bounds__(256) k_correction(int ni, int nj, int nk, View
     block_reduce<2>(v, mx, partials);
 }
 
+// Correction in difference form, for a linear coarse propagator G (this one is: the
+// operator is linear in q and nu(t) does not depend on q), where
+//   G(U_n^{k+1}) - G(U_n^k) = G(U_n^{k+1} - U_n^k) = G(d_n):
+//   q = fine + G(d_n)  (gd = G(d_n) in the coarse storage type; a null gd.p stands for
+//   G(0) = 0, slice 0, whose start state never changes), d_{n+1} = q - dst_old into dnext
+//   (skipped when dnext.p is null), dst <- q, and partial sums of d_{n+1}^2 and q^2.
+// Rounding of a reduced-precision G then scales with |d|, which vanishes as Parareal
+// converges, instead of with |U|, which does not.
+template <typename SG>
+__global__ void __launch_bounds__(256) k_correction_delta(int ni, int nj, int nk, View gd, View fine, View dst,
+                                                          View dnext, double* partials)
+{
+#pragma clang fp contract(off)
+    const long long n = (long long)ni * nj * nk;
+    double d2 = 0., q2 = 0.;
+    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long long)gridDim.x * 256) {
+        const int i = (int)(p % ni), j = (int)((p / ni) % nj), k = (int)(p / ((long long)ni * nj));
+        const double g = gd.p ? (double)Store<SG>::load((const SG*)gd.p + i + j * gd.sj + k * gd.sk) : 0.;
+        const double b = ((const double*)fine.p)[i + j * fine.sj + k * fine.sk];
+        const double q = b + g;
+        double* d = (double*)dst.p + i + j * dst.sj + k * dst.sk;
+        const double diff = q - *d;
+        *d = q;
+        if (dnext.p) ((double*)dnext.p)[i + j * dnext.sj + k * dnext.sk] = diff;
+        d2 += diff * diff;
+        q2 += q * q;
+    }
+    double v[2] = {d2, q2};
+    const bool mx[2] = {false, false};
+    block_reduce<2>(v, mx, partials);
+}
+
 __global__ void __launch_bounds__(256) k_norms(int ni, int nj, int nk, View a, View b, double* partials)
 {
     const long long n = (long long)ni * nj * nk;
@@ -243,6 +275,22 @@ int launch_correction(hipStream_t s, int ni, int nj, int nk, int g_dtype, View c
     else
         hipLaunchKernelGGL(k_correction<bf16>, dim3(nblocks), dim3(256), 0, s, ni, nj, nk, cnew, cold, fine, dst,
                            partials);
+    SP_HIP(hipGetLastError());
+    return SP_OK;
+}
+
+int launch_correction_delta(hipStream_t s, int ni, int nj, int nk, int g_dtype, View gd, View fine, View dst,
+                            View dnext, double* partials, int nblocks)
+{
+    if (g_dtype == SP_F64)
+        hipLaunchKernelGGL(k_correction_delta<double>, dim3(nblocks), dim3(256), 0, s, ni, nj, nk, gd, fine, dst,
+                           dnext, partials);
+    else if (g_dtype == SP_F32)
+        hipLaunchKernelGGL(k_correction_delta<float>, dim3(nblocks), dim3(256), 0, s, ni, nj, nk, gd, fine, dst,
+                           dnext, partials);
+    else
+        hipLaunchKernelGGL(k_correction_delta<bf16>, dim3(nblocks), dim3(256), 0, s, ni, nj, nk, gd, fine, dst,
+                           dnext, partials);
     SP_HIP(hipGetLastError());
     return SP_OK;
 }

@@ -23,11 +23,16 @@ class Parareal:
     coarse_dtype: "f64", "f32" or "bf16" (storage; f32 arithmetic) for the RK4 coarse.
     tol: stop once the relative L2 increment of the slice states is <= tol (0 = run
     exactly kmax iterations, as the reference does, Parareal.h:116).
+    autotune: time the fused kernel's tile shapes for F and G on these sizes at creation.
+    correction: "standard" (the reference's G(U_n^{k+1}) + F(U_n^k) - G(U_n^k)), "delta"
+    (F(U_n^k) + G(U_n^{k+1} - U_n^k), the same for this linear G, with G's rounding
+    proportional to the increment) or "auto" (delta for f32 / bf16 coarse storage).
     """
 
     def __init__(self, propagator: Convection, initial: ConvectionField, solution: ConvectionField,
                  conf: RuntimeConfiguration, *, coarse: str = "euler", coarse_dtype: str = "f64",
-                 substeps: int = 1, tol: float = 0.0, use_graphs: bool = False):
+                 substeps: int = 1, tol: float = 0.0, use_graphs: bool = False, autotune: bool = False,
+                 correction: str = "auto"):
         self.prop = propagator
         self.qinitial = initial
         self.q = solution
@@ -44,6 +49,10 @@ class Parareal:
         cfg.tol = tol
         cfg.async_schedule = 1 if conf.async_ else 0
         cfg.use_graphs = 1 if use_graphs else 0
+        cfg.autotune = 1 if autotune else 0
+        if correction == "auto":
+            correction = "standard" if coarse_dtype == "f64" else "delta"
+        cfg.delta_correction = {"standard": 0, "delta": 1}[correction]
         self.cfg = cfg
         h = C.c_void_p()
         L.call("sp_parareal_create", propagator.h, C.byref(cfg), C.byref(h))

@@ -1,0 +1,154 @@
+"""The benchmark configurations as GPU parity cases.
+
+cfg1 (32^3, 4 slices, all FP64, T = 0.5): in full, every Parareal iterate against the
+  oracle's Parareal, and the serial fine solution against the oracle's.
+cfg2 (128^3, 16 slices, FP64): on a shortened horizon (8 fine steps per slice, fine/coarse
+  step ratio kept), every iterate against the oracle.
+cfg3 (256^3, 32 slices, FP64 F, FP32 G) and cfg4 (384^3, 64 slices, FP64 F, BF16-storage G
+  with 2 sub-steps, stop at a 1e-6 relative increment): at their full grid and slice
+  count on a shortened horizon, through properties that do not depend on the size:
+  the fused F agrees with the oracle over a step at that size (1e-12), the error of the
+  iterates against the GPU's serial FP64 fine solution falls with k to the FP64 floor
+  (the reduced-precision G runs in the difference form of the correction), k = S
+  reproduces that solution (cfg3), and the stopping rule ends the run with the error at
+  the level of the last increment (cfg4).
+Random draws (Gaussian centres, widths, amplitudes, velocities) come from SplitMix64 with
+seed 0, as in the benchmark.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle as O  # noqa: E402
+from stencilparareal_amd.config import RuntimeConfiguration, workload  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def sp():
+    import stencilparareal_amd as spm
+    from stencilparareal_amd import field
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    field.Context.default(0)
+    return spm
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def conf_of(w, stepsF=None, stepsG=None, T=None, kmax=None):
+    return RuntimeConfiguration(gridSize=w.N, endTime=T or w.T, timeStepsFine=stepsF or w.stepsF,
+                                timeStepsCoarse=stepsG or w.stepsG, nu0=w.nu, nufreq=0.0, cx=w.c[0], cy=w.c[1],
+                                cz=w.c[2], kmax=kmax or w.kmax, async_=False, timeSlices=w.slices)
+
+
+def gauss0(w):
+    g = w.gauss
+    return O.fill_gaussians(w.N, g.xc, g.yc, g.zc, g.sigma, g.amp)
+
+
+def run_iterates(sp, conv, conf, w, u0_dev, ref_dev, **kw):
+    sol = conv.field("q")
+    pr = sp.Parareal(conv, u0_dev, sol, conf, coarse=w.coarse, coarse_dtype=w.coarse_dtype,
+                     substeps=w.substeps, **kw)
+    pr.begin()
+    errs, sols, incs = [], [], []
+    for k in range(conf.kmax + 1):
+        if k:
+            incs.append(pr.iterate())
+        s = pr.solution()
+        errs.append(sp.rel_l2(s, ref_dev) if ref_dev is not None else None)
+        sols.append(s.get() if ref_dev is None else None)
+    return errs, sols, incs, pr
+
+
+@pytest.mark.parametrize("idx,short", [(1, False), (2, True)], ids=["cfg1-full", "cfg2-short"])
+def test_config_iterates_match_oracle(sp, idx, short):
+    w = workload(idx)
+    if short:  # 8 fine steps per slice, same fine step and fine/coarse ratio (8 -> 1 step)
+        S = w.slices
+        dtF = w.T / w.stepsF
+        conf = conf_of(w, stepsF=8 * S, stepsG=S, T=8 * S * dtF, kmax=2)
+    else:
+        conf = conf_of(w)
+    N, S = w.N, w.slices
+    u0 = gauss0(w)
+    oc = O.OConvection(N, N, N, conf.dx(), w.nu, 0.0, *w.c)
+    _, iters = oc.parareal(u0, S, conf.kmax, conf.endTime, conf.dtFine(), conf.timeStepsFinePerTimeSlice(),
+                           conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(), w.coarse)
+    serial = oc.serial(u0, conf.dtFine(), conf.timeStepsFinePerTimeSlice() * S)
+    conv = sp.Convection(N, N, N, conf.dx(), w.nu, 0.0, *w.c)
+    init = conv.field("qinitial")
+    init.set(u0)
+    _, sols, _, pr = run_iterates(sp, conv, conf, w, init, None)
+    for k in range(conf.kmax + 1):
+        assert relerr(sols[k], iters[k]) <= 1e-12, (k, relerr(sols[k], iters[k]))
+    print(f"{w.name}: rel L2 error vs serial fine per iterate:",
+          ["%.3e" % O.rel_l2(iters[k], serial) for k in range(conf.kmax + 1)])
+    ser = conv.field("serial")
+    pr.q = ser
+    pr.DoSerial()
+    assert relerr(ser.get(), serial) <= 1e-12
+
+
+def fine_step_matches_oracle(sp, conv, w, init, dt):
+    # one fused FP64 step of one slice at this size against the oracle
+    out = conv.field("f1")
+    conv.DoRK4(init, out, 0.0, dt, 1)
+    N = w.N
+    ref = O.OConvection(N, N, N, 1.0 / N, w.nu, 0.0, *w.c).rk4(init.get(), 0.0, dt, 1)
+    e = relerr(out.get(), ref)
+    assert e <= 1e-12, e
+    del out
+
+
+def test_cfg3_full_size_short_horizon(sp):
+    w = workload(3)
+    S, N = w.slices, w.N
+    dtF = w.T / w.stepsF
+    conf = conf_of(w, stepsF=4 * S, stepsG=S, T=4 * S * dtF, kmax=S)
+    conv = sp.Convection(N, N, N, conf.dx(), w.nu, 0.0, *w.c)
+    init, ref = conv.field("qinitial"), conv.field("serial")
+    g = w.gauss
+    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    fine_step_matches_oracle(sp, conv, w, init, conf.dtFine())
+    sp.Parareal(conv, init, ref, conf, coarse="rk4").DoSerial()
+    errs, _, incs, pr = run_iterates(sp, conv, conf, w, init, ref)
+    print("cfg3 rel L2 per iterate:", ["%.2e" % e for e in errs[:8]], "... k=S:", "%.2e" % errs[-1])
+    # FP32 G in the difference form: the error falls geometrically to the FP64 floor
+    assert errs[1] < 1e-2 * errs[0]
+    assert all(b < a or b <= 1e-13 for a, b in zip(errs, errs[1:]))
+    assert errs[4] <= 1e-12
+    assert errs[-1] <= 1e-12  # k = S: the serial FP64 fine solution, whatever G is
+    del pr
+
+
+def test_cfg4_full_size_short_horizon_stops_at_tol(sp):
+    w = workload(4)
+    S, N = w.slices, w.N
+    dtF = w.T / w.stepsF
+    conf = conf_of(w, stepsF=2 * S, stepsG=S, T=2 * S * dtF, kmax=8)
+    conv = sp.Convection(N, N, N, conf.dx(), w.nu, 0.0, *w.c)
+    init, ref, sol = conv.field("qinitial"), conv.field("serial"), conv.field("q")
+    g = w.gauss
+    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    fine_step_matches_oracle(sp, conv, w, init, conf.dtFine())
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype="bf16", substeps=2, tol=1e-6,
+                     use_graphs=True)
+    pr.q = ref
+    pr.DoSerial()
+    pr.q = sol
+    rep = pr.DoParallel()
+    incs = list(rep.increment[:rep.iterations])
+    e = sp.rel_l2(sol, ref)
+    print(f"cfg4: {rep.iterations} iterations, increments", ["%.2e" % x for x in incs], "final rel L2 %.2e" % e)
+    assert rep.iterations <= 8
+    assert incs[-1] <= 1e-6 or rep.iterations == 8
+    assert all(b < a or b <= 1e-13 for a, b in zip(incs, incs[1:]))
+    # the remaining error is of the order of the last increment (Parareal contracts by far
+    # more than 10x per iteration here)
+    assert e <= 10 * incs[-1] + 1e-12
+    del pr

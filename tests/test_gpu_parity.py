@@ -181,6 +181,26 @@ def test_thin_k_runs_reference_order_path(sp, nk):
         conv.DoRK4(a, conv.field(dtype="f32"), 0.0, 1e-4, 1)
 
 
+@pytest.mark.parametrize("nslices,dtype", [(1, "f64"), (6, "f64"), (6, "f32")])
+def test_autotuned_tile_is_used_and_exact(sp, nslices, dtype):
+    N = 48
+    conv = sp.Convection(N, N, N, 1. / N, 0.01, 0.0, *C3)
+    us = conv.autotune(nslices, dtype, dtype)
+    assert 0 < us < 1e6
+    t = conv.get_tile(nslices, dtype == "f64")
+    assert t["variant"] in ("scalar", "pair") and t["tx"] > 0 and t["kchunk"] >= 0
+    if dtype == "f64":
+        q0 = smooth_field(N, 11)
+        a, b = conv.field(), conv.field()
+        a.set(q0)
+        conv.DoRK4(a, b, 0.0, 1e-3, 4)  # one slice: uses the tile tuned for 1 slice, if any
+        ref = O.OConvection(N, N, N, 1. / N, 0.01, 0.0, *C3).rk4(q0, 0.0, 1e-3, 4)
+        assert relerr(b.get(), ref) <= 1e-12
+    # an explicit tile still wins over the tuned one
+    conv.set_tile(16, 16, 256, 0, variant="scalar")
+    assert conv.get_tile(nslices, dtype == "f64")["tx"] == 16
+
+
 def test_fused_rk4_long_run(sp):
     N, steps = 32, 300
     q0 = O.fill_gaussians(N, [0.3, 0.7], [0.5, 0.2], [0.6, 0.9], [0.08, 0.05], [1.0, 0.7])
@@ -311,6 +331,16 @@ def test_parareal_fused_vs_oracle(sp, coarse, graphs):
     assert relerr(sol.get(), serial) <= 1e-12
 
 
+def test_parareal_autotuned_vs_oracle(sp):
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4")
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", use_graphs=True, autotune=True)
+    rep = pr.DoParallel()
+    assert rep.iterations == conf.kmax
+    assert relerr(sol.get(), iters[conf.kmax]) <= 1e-12
+
+
 @pytest.mark.parametrize("gdt", ["f32", "bf16"])
 def test_parareal_mixed_precision_coarse_reaches_serial_fine(sp, gdt):
     conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4", S=4, kmax=4)
@@ -327,6 +357,44 @@ def test_parareal_mixed_precision_coarse_reaches_serial_fine(sp, gdt):
     # a slice's input stops changing), to the fused fine propagator's tolerance
     assert errs[-1] <= 1e-12
     assert errs[-1] < 1e-3 * errs[0]
+
+
+def test_parareal_delta_correction_fp64_matches_oracle(sp):
+    # F(U^k) + G(U^{k+1} - U^k) equals the reference's update for this linear G
+    conf, conv, u0, iters, _ = _parareal_case(sp, "fused", "rk4", S=6, kmax=5)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", correction="delta", use_graphs=True)
+    pr.begin()
+    assert relerr(pr.solution().get(), iters[0]) <= 1e-12
+    for k in range(1, conf.kmax + 1):
+        pr.iterate()
+        assert relerr(pr.solution().get(), iters[k]) <= 1e-12, k
+
+
+@pytest.mark.parametrize("gdt", ["f32", "bf16"])
+def test_delta_correction_removes_reduced_precision_plateau(sp, gdt):
+    # With G in f32 / bf16 storage the reference's update subtracts two independently
+    # rounded coarse values, so the iterates stall at the storage rounding; the difference
+    # form rounds G's increment instead and keeps converging to the FP64 fine solution.
+    conf, conv, u0, _, serial = _parareal_case(sp, "fused", "rk4", N=16, S=8, kmax=6)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    errs = {}
+    for form in ("standard", "delta"):
+        pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=gdt, correction=form)
+        pr.begin()
+        e = [O.rel_l2(pr.solution().get(), serial)]
+        for _ in range(conf.kmax):
+            pr.iterate()
+            e.append(O.rel_l2(pr.solution().get(), serial))
+        errs[form] = e
+        del pr
+    print(gdt, {f: ["%.2e" % x for x in v] for f, v in errs.items()})
+    d, st = errs["delta"], errs["standard"]
+    assert all(b < a or b <= 1e-13 for a, b in zip(d, d[1:]))
+    assert d[-1] <= 1e-2 * st[-1]
+    assert d[-1] <= 1e-9
 
 
 def test_parareal_tolerance_stops_early(sp):
