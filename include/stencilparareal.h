@@ -230,6 +230,56 @@ int sp_parareal_run(sp_parareal* pr, const sp_layout* l0, const void* u0, const 
 int sp_parareal_serial(sp_parareal* pr, const sp_layout* l0, const void* u0, const sp_layout* lq,
                        void* q);
 
+/* ---- batches of independent Parareal problems ------------------------------------ */
+/* No counterpart in the reference, which runs one problem per MPI job.  N independent
+ * problems (own grid, slice count, coefficients, horizon) run as one Parareal whose
+ * launches each cover all of them: one grouped fused-RK4 launch per fine step (per tile
+ * shape in use), one grouped launch per coarse sub-step at each position of the coarse
+ * chains, one grouped correction launch per position.  F is FP64; G is RK4 in SP_F64,
+ * SP_F32 or SP_BF16 storage; the correction is the difference form (see
+ * sp_parareal_cfg.delta_correction).  Constant diffusion (nufreq = 0). */
+typedef struct sp_problem {
+    int n;                        /* grid points per dimension (n^3), even, >= 8 */
+    int slices;                   /* time slices S_p */
+    double nu, cx, cy, cz;        /* diffusion, advection velocity */
+    double end_time;              /* T_p */
+    int steps_fine, steps_coarse; /* totals over [0, T_p], rounded up to multiples of S_p */
+} sp_problem;
+
+typedef struct sp_batch_cfg {
+    int kmax;            /* maximum iterations */
+    int coarse_dtype;    /* SP_F64, SP_F32 or SP_BF16 */
+    int coarse_substeps; /* >= 1 sub-steps per coarse step */
+    double tol;          /* stop once every problem's relative L2 increment <= tol; 0: run kmax */
+    int use_graphs;      /* 1: capture one iteration as a hipGraph and replay it */
+    int autotune;        /* 1: autotune the fused tiles once per (grid, slice count) */
+} sp_batch_cfg;
+
+typedef struct sp_batch_report {
+    int iterations;
+    double increment[SP_MAX_ITERS]; /* largest relative L2 increment over the problems */
+    double t_init, t_total;         /* seconds: initial coarse chains; whole run */
+    double lattice_updates;         /* fine + coarse point updates of the run */
+} sp_batch_report;
+
+typedef struct sp_batch_info {
+    int steps_fine_per_slice, steps_coarse_per_slice;
+    double dt_fine, dt_coarse;
+    int fine_variant, fine_tx, fine_ty, fine_nthreads, fine_kchunk; /* fine-sweep tile */
+} sp_batch_info;
+
+typedef struct sp_batch sp_batch;
+int sp_batch_create(sp_ctx* ctx, int nprob, const sp_problem* probs, const sp_batch_cfg* cfg, sp_batch** out);
+int sp_batch_destroy(sp_batch* b);
+/* u0[p]: problem p's initial field, q[p]: receives its last slice's state at T_p; both
+ * compact SP_F64 device arrays of n_p^3 doubles.  Synchronous. */
+int sp_batch_run(sp_batch* b, void* const* u0, void* const* q, sp_batch_report* rep);
+/* Problem p's relative L2 increments of the last run, one per iteration. */
+int sp_batch_increments(const sp_batch* b, int p, double* inc);
+int sp_batch_problem_info(const sp_batch* b, int p, sp_batch_info* info);
+/* Problem p's serial FP64 fine solution: DoRK4(u0 -> q, 0, dtF_p, tsF_p * S_p). */
+int sp_batch_serial(sp_batch* b, int p, const void* u0, void* q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ This package is the Python host mirror of the reference interfaces:
 
     Convection(isize, jsize, ksize, dx, nu0, nufreq, cx, cy, cz).DoRK4(in, out, t0, dt, n)
     Parareal(convection, initial, solution, conf).DoParallel() / DoSerial()
+    Batch([Problem(...), ...], kmax=...).run(u0_fields, q_fields)   # uneven problem batches
 
 There is no CPU fallback: importing the compute classes requires the built library.
 """
@@ -23,6 +24,9 @@ def __getattr__(name):
     if name in ("Parareal",):
         from .parareal import Parareal
         return Parareal
+    if name in ("Batch", "Problem"):
+        from . import batch
+        return getattr(batch, name)
     if name in ("ConvectionField", "Context", "periodic_fill", "norms", "error_reference", "rel_l2",
                 "fill_exact", "fill_gaussians"):
         from . import field

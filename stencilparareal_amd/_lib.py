@@ -38,6 +38,8 @@ EXPORTS = [
     "sp_parareal_update", "sp_norms", "sp_fill_exact", "sp_fill_gaussians", "sp_parareal_create",
     "sp_parareal_destroy", "sp_parareal_begin", "sp_parareal_iterate", "sp_parareal_solution",
     "sp_parareal_run", "sp_parareal_serial",
+    "sp_batch_create", "sp_batch_destroy", "sp_batch_run", "sp_batch_increments", "sp_batch_problem_info",
+    "sp_batch_serial",
 ]
 
 
@@ -67,6 +69,28 @@ class PararealReport(C.Structure):
                 ("dt_coarse", C.c_double), ("increment", C.c_double * SP_MAX_ITERS),
                 ("t_init", C.c_double), ("t_fine", C.c_double * SP_MAX_ITERS),
                 ("t_coarse", C.c_double * SP_MAX_ITERS), ("t_total", C.c_double)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("n", C.c_int), ("slices", C.c_int), ("nu", C.c_double), ("cx", C.c_double),
+                ("cy", C.c_double), ("cz", C.c_double), ("end_time", C.c_double),
+                ("steps_fine", C.c_int), ("steps_coarse", C.c_int)]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("kmax", C.c_int), ("coarse_dtype", C.c_int), ("coarse_substeps", C.c_int),
+                ("tol", C.c_double), ("use_graphs", C.c_int), ("autotune", C.c_int)]
+
+
+class BatchReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("increment", C.c_double * SP_MAX_ITERS), ("t_init", C.c_double),
+                ("t_total", C.c_double), ("lattice_updates", C.c_double)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("steps_fine_per_slice", C.c_int), ("steps_coarse_per_slice", C.c_int),
+                ("dt_fine", C.c_double), ("dt_coarse", C.c_double), ("fine_variant", C.c_int),
+                ("fine_tx", C.c_int), ("fine_ty", C.c_int), ("fine_nthreads", C.c_int), ("fine_kchunk", C.c_int)]
 
 
 _vp = C.c_void_p
@@ -114,6 +138,12 @@ _SIGS = {
     "sp_parareal_solution": (C.c_int, [_vp, _lp, _vp]),
     "sp_parareal_run": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.POINTER(PararealReport)]),
     "sp_parareal_serial": (C.c_int, [_vp, _lp, _vp, _lp, _vp]),
+    "sp_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(Problem), C.POINTER(BatchCfg), C.POINTER(_vp)]),
+    "sp_batch_destroy": (C.c_int, [_vp]),
+    "sp_batch_run": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(BatchReport)]),
+    "sp_batch_increments": (C.c_int, [_vp, C.c_int, _dp]),
+    "sp_batch_problem_info": (C.c_int, [_vp, C.c_int, C.POINTER(BatchInfo)]),
+    "sp_batch_serial": (C.c_int, [_vp, C.c_int, _vp, _vp]),
 }
 
 _lib = None

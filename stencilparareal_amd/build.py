@@ -31,8 +31,9 @@ SOURCES = {
     "sp_staged.hip": ["-ffp-contract=off"],
     "sp_misc.hip": ["-ffp-contract=off"],
     "sp_fused.hip": [],
+    "sp_batch.hip": [],
 }
-HEADERS = ["sp_internal.hpp", os.path.join("..", "..", "include", "stencilparareal.h")]
+HEADERS = ["sp_internal.hpp", "sp_handles.hpp", os.path.join("..", "..", "include", "stencilparareal.h")]
 
 
 def hipcc() -> str:

@@ -198,6 +198,41 @@ def workload(idx: int, seed: int = 0) -> Workload:
     raise ValueError(idx)
 
 
+@dataclass
+class UnevenProblem:
+    n: int
+    slices: int
+    nu: float
+    c: tuple
+    end_time: float
+    steps_fine: int
+    steps_coarse: int
+    gauss: Gaussians
+
+
+def uneven_batch(count: int = 24, seed: int = 0, sizes=(64, 96, 128, 160), slices=(8, 16, 24),
+                 coarse_per_slice: int = 2, ratio: int = 16) -> list[UnevenProblem]:
+    """The uneven batch: `count` independent problems, grid n from `sizes`, nu log-uniform
+    in [1e-3, 1e-2], c uniform in [-1, 1]^3, slices from `slices`, all drawn from
+    SplitMix64(seed).  Time steps (the configuration leaves them open): the coarse RK4 step
+    is half the largest stable step for that grid, nu and c (a scan of the operator's
+    symbol), the fine step is `ratio` times smaller, each slice spans `coarse_per_slice`
+    coarse steps, so T_p = S_p * coarse_per_slice * dt_coarse.  Initial field: 4 Gaussians
+    (centres uniform in [0,1]^3, sigma in [0.04, 0.1], amplitude in [0.5, 1])."""
+    u = uniform_stream(seed)
+    out = []
+    for i in range(count):
+        n = sizes[min(int(next(u) * len(sizes)), len(sizes) - 1)]
+        nu = 10.0 ** (-3.0 + next(u))
+        c = tuple(-1.0 + 2.0 * next(u) for _ in range(3))
+        S = slices[min(int(next(u) * len(slices)), len(slices) - 1)]
+        dtG = 0.5 * max_stable_rk4_dt(n, nu, c)
+        T = S * coarse_per_slice * dtG
+        out.append(UnevenProblem(n, S, nu, c, T, S * coarse_per_slice * ratio, S * coarse_per_slice,
+                                 random_gaussians(4, seed * 1000 + 17 + i)))
+    return out
+
+
 def rounded_steps(total: int, slices: int) -> int:
     return total if total % slices == 0 else (total // slices + 1) * slices
 
@@ -205,7 +240,7 @@ def rounded_steps(total: int, slices: int) -> int:
 def max_stable_rk4_dt(N: int, nu: float, c) -> float:
     """Largest dt with |R(dt*lambda)| <= 1 for every Fourier mode of the 4th-order operator,
     by a scan over the symbol (used to pick coarse sub-steps)."""
-    th = np.linspace(0, math.pi, 65)
+    th = np.linspace(0, math.pi, 33)
     tx, ty, tz = np.meshgrid(th, th, th, indexing="ij")
     dx = 1.0 / N
 
@@ -213,9 +248,10 @@ def max_stable_rk4_dt(N: int, nu: float, c) -> float:
         return (-1j * cd * (8 * np.sin(t) - np.sin(2 * t)) / (6 * dx)
                 + nu * (-2 * np.cos(2 * t) + 32 * np.cos(t) - 30) / (12 * dx * dx))
 
-    L_ = lam(tx, abs(c[0])) + lam(ty, abs(c[1])) + lam(tz, abs(c[2]))
-    lo, hi = 0.0, 1.0
-    for _ in range(60):
+    L_ = (lam(tx, abs(c[0])) + lam(ty, abs(c[1])) + lam(tz, abs(c[2]))).ravel()
+    # the RK4 stability region lies within |z| < 2.9, so dt < 2.9 / max|lambda|
+    lo, hi = 0.0, 2.9 / np.abs(L_).max()
+    for _ in range(48):
         mid = 0.5 * (lo + hi)
         z = mid * L_
         R = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24

@@ -33,6 +33,7 @@
 //  * The plane for iteration t+1 is loaded into registers while iteration t computes.
 
 #include <algorithm>
+#include <cstring>
 
 #include "sp_internal.hpp"
 
@@ -61,6 +62,13 @@ struct FusedArgs {
     int tiles_x, tiles;
     Coefs c;                 // used when cdev == nullptr
     const Coefs* cdev;       // per-slice coefficients (time-dependent nu), or nullptr
+};
+
+// A grouped launch: n argument records in device memory and their first block indices.
+struct GroupArgs {
+    const FusedArgs* __restrict__ args;
+    const int* __restrict__ first;
+    int n;
 };
 
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -241,7 +249,7 @@ __device__ __forceinline__ int xcd_block(int b, int n)
 }
 
 template <typename SI, typename SO, typename T, int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
+__device__ __forceinline__ void rk4_scalar_body(const FusedArgs& a, const int bid)
 {
     using G1 = Geo<TX, TY, 1>;
     using G2 = Geo<TX, TY, 2>;
@@ -264,7 +272,6 @@ __global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
     const int tid = threadIdx.x;
     const int ni = a.ni, nj = a.nj, nk = a.nk;
     // block -> (tile, k chunk, slice); tiles fastest
-    const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int tile = bid % a.tiles;
     const int chunk = (bid / a.tiles) % a.chunks;
     const int slice = bid / (a.tiles * a.chunks);
@@ -558,7 +565,7 @@ __device__ __forceinline__ void pair_qidx(int (&same)[M], int (&next)[M])
 //     LDS plane Q3 beside its output; stage 3 reads q from Q3 with its input plane, keeps
 //     it 2 planes in registers and forwards it through Q4 to stage 4 the same way.
 template <int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
+__device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 {
     using G1 = Geo<TX, TY, 1>;
     using G2 = Geo<TX, TY, 2>;
@@ -581,7 +588,6 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
 
     const int tid = threadIdx.x;
     const int ni = a.ni, nj = a.nj, nk = a.nk;
-    const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int tile = bid % a.tiles;
     const int chunk = (bid / a.tiles) % a.chunks;
     const int slice = bid / (a.tiles * a.chunks);
@@ -739,6 +745,51 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
     }
 }
 
+// One launch over one batch of slices.
+template <typename SI, typename SO, typename T, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
+{
+    rk4_scalar_body<SI, SO, T, TX, TY, NT>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
+{
+    rk4_pair_body<TX, TY, NT>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// One launch over several independent batches (problems of different sizes, steps and
+// coefficients): entry e owns blocks [first[e], first[e+1]); a block finds its entry by a
+// scalar scan (entries are few) and runs the same body on that entry's arguments.
+__device__ __forceinline__ int group_entry(const GroupArgs& g, int bid)
+{
+    int e = 0;
+    while (e + 1 < g.n && g.first[e + 1] <= bid) ++e;
+    return e;
+}
+
+// The records are read through the constant address space: with a block-uniform index
+// the loads are scalar, so the arguments live in scalar registers as kernel arguments do.
+typedef __attribute__((address_space(4))) const FusedArgs ConstArgs;
+
+template <typename SI, typename SO, typename T, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_rk4_fused_grouped(GroupArgs g)
+{
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int e = group_entry(g, bid);
+    ConstArgs* rec = (ConstArgs*)g.args + e;
+    rk4_scalar_body<SI, SO, T, TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
+}
+
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_rk4_fused_pair_grouped(GroupArgs g)
+{
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int e = group_entry(g, bid);
+    ConstArgs* rec = (ConstArgs*)g.args + e;
+    rk4_pair_body<TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
+}
+
 }  // namespace
 
 Coefs make_coefs(double nu, double cx, double cy, double cz, double dx, double h)
@@ -843,7 +894,69 @@ bool pair_ok(const FusedStep& st, View in, View out)
            even(out.slice) && ((uintptr_t)in.p & 15) == 0 && ((uintptr_t)out.p & 15) == 0;
 }
 
+template <typename SI, typename SO, typename T>
+int launch_grouped_typed(hipStream_t s, const FusedTile& tile, GroupArgs g, long long nblocks)
+{
+    const dim3 grid((unsigned)nblocks);
+#define SP_FUSED_CASE(TX_, TY_, NT_)                                                                     \
+    if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                                            \
+        hipLaunchKernelGGL((k_rk4_fused_grouped<SI, SO, T, TX_, TY_, NT_>), grid, dim3(NT_), 0, s, g);   \
+        SP_HIP(hipGetLastError());                                                                       \
+        return SP_OK;                                                                                    \
+    }
+    SP_FUSED_CASE(32, 32, 1024)
+    SP_FUSED_CASE(32, 24, 1024)
+    SP_FUSED_CASE(24, 24, 1024)
+    SP_FUSED_CASE(32, 16, 512)
+    SP_FUSED_CASE(16, 16, 256)
+#undef SP_FUSED_CASE
+    return fail(SP_ERR_UNSUPPORTED, "grouped fused RK4: unsupported tile shape");
+}
+
 }  // namespace
+
+size_t fused_record_size() { return sizeof(FusedArgs); }
+
+int fused_record_fill(void* rec, const FusedStep& st, const FusedTile& tile, int in_dtype, View in, int out_dtype,
+                      View out, long long* nblocks)
+{
+    SP_CHECK(st.nk >= kFusedMinK, SP_ERR_UNSUPPORTED, "grouped fused RK4: ksize below the fused kernel's minimum");
+    FusedArgs a;
+    SP_TRY(fill_args(a, st, tile, in, out, dtype_size(in_dtype), dtype_size(out_dtype)));
+    std::memcpy(rec, &a, sizeof(a));
+    *nblocks = (long long)nblocks_of(a, st.nslices);
+    return SP_OK;
+}
+
+bool fused_record_pair_ok(const FusedStep& st, View in, View out) { return pair_ok(st, in, out); }
+
+int launch_rk4_grouped(hipStream_t s, const FusedTile& tile, int in_dtype, int out_dtype, const void* records,
+                       const int* first, int n, long long nblocks)
+{
+    SP_CHECK(n >= 1 && nblocks >= 1 && nblocks <= 0x7fffffffLL, SP_ERR_INVALID, "grouped fused RK4: empty or too large");
+    const GroupArgs g{(const FusedArgs*)records, first, n};
+    if (tile.variant == SP_KERNEL_PAIR) {
+        SP_CHECK(in_dtype == SP_F64 && out_dtype == SP_F64, SP_ERR_UNSUPPORTED, "grouped pair kernel is FP64 only");
+        const dim3 grid((unsigned)nblocks);
+#define SP_PAIR_CASE(TX_, TY_, NT_)                                                               \
+        if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                                  \
+            hipLaunchKernelGGL((k_rk4_fused_pair_grouped<TX_, TY_, NT_>), grid, dim3(NT_), 0, s, g); \
+            SP_HIP(hipGetLastError());                                                             \
+            return SP_OK;                                                                          \
+        }
+        SP_PAIR_CASE(32, 32, 512)
+        SP_PAIR_CASE(32, 16, 512)
+        SP_PAIR_CASE(16, 16, 256)
+#undef SP_PAIR_CASE
+        return fail(SP_ERR_UNSUPPORTED, "grouped fused RK4 (pair): unsupported tile shape");
+    }
+    if (in_dtype == SP_F64 && out_dtype == SP_F64) return launch_grouped_typed<double, double, double>(s, tile, g, nblocks);
+    if (in_dtype == SP_F64 && out_dtype == SP_F32) return launch_grouped_typed<double, float, float>(s, tile, g, nblocks);
+    if (in_dtype == SP_F32 && out_dtype == SP_F32) return launch_grouped_typed<float, float, float>(s, tile, g, nblocks);
+    if (in_dtype == SP_F64 && out_dtype == SP_BF16) return launch_grouped_typed<double, bf16, float>(s, tile, g, nblocks);
+    if (in_dtype == SP_BF16 && out_dtype == SP_BF16) return launch_grouped_typed<bf16, bf16, float>(s, tile, g, nblocks);
+    return fail(SP_ERR_UNSUPPORTED, "grouped fused RK4: unsupported dtype pair");
+}
 
 bool fused_tile_supported(const FusedTile& t, int /*compute_f64*/)
 {

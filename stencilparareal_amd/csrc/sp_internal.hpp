@@ -153,6 +153,16 @@ int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile, 
                      int out_dtype, View out);
 bool fused_tile_supported(const FusedTile& t, int compute_f64);
 FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64);
+// Grouped launches: one launch over several batches with their own sizes, views, steps and
+// coefficients.  fused_record_fill writes the kernel's argument record for one batch (of
+// fused_record_size() bytes) and its block count; the caller uploads the records and the
+// prefix sums of the block counts (first block of each record) to device memory.
+size_t fused_record_size();
+int fused_record_fill(void* rec, const FusedStep& st, const FusedTile& tile, int in_dtype, View in, int out_dtype,
+                      View out, long long* nblocks);
+bool fused_record_pair_ok(const FusedStep& st, View in, View out);
+int launch_rk4_grouped(hipStream_t s, const FusedTile& tile, int in_dtype, int out_dtype, const void* records,
+                       const int* first, int n, long long nblocks);
 // Tile shapes and k chunks the autotuner times for this size and batch.
 void fused_candidates(int ni, int nj, int nk, int nslices, int compute_f64, std::vector<FusedTile>& out);
 
@@ -166,6 +176,13 @@ int launch_correction(hipStream_t s, int ni, int nj, int nk, int g_dtype, View c
                       View dst, double* partials, int nblocks);
 int launch_correction_delta(hipStream_t s, int ni, int nj, int nk, int g_dtype, View gd, View fine, View dst,
                             View dnext, double* partials, int nblocks);
+// One slice's difference-form correction as a record of a grouped launch (device memory).
+struct CorrRecord {
+    View gd, fine, dst, dnext;
+    int ni, nj, nk;
+    double* partials;  // nb blocks x 2 partial sums
+};
+int launch_correction_delta_grouped(hipStream_t s, int g_dtype, const CorrRecord* recs, int n, int nb);
 int launch_norms(hipStream_t s, int ni, int nj, int nk, View a, View b, double* partials, int nblocks);
 int launch_fill_exact(hipStream_t s, const sp_layout& l, void* base, double nu0, double nufreq, double cx,
                       double cy, double cz, double t);

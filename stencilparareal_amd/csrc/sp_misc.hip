@@ -67,7 +67,8 @@ __global__ void __launch_bounds__(256) k_update(int ni, int nj, int nk, View q, 
 
 // ---- block reduction helpers ---------------------------------------------------------------
 template <int NV>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], const bool (&is_max)[NV], double* out)
+__device__ __forceinline__ void block_reduce(double (&v)[NV], const bool (&is_max)[NV], double* out,
+                                             int block = -1)
 {
     __shared__ double sh[NV][256];
     const int t = threadIdx.x;
@@ -84,7 +85,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], const bool (&is_ma
     }
     if (t == 0) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) out[blockIdx.x * NV + q] = sh[q][0];
+        for (int q = 0; q < NV; ++q) out[(block < 0 ? (int)blockIdx.x : block) * NV + q] = sh[q][0];
     }
 }
 
@@ -129,17 +130,18 @@ __global__ void __launch_bounds__(256) k_correction(int ni, int nj, int nk, View
 // Rounding of a reduced-precision G then scales with |d|, which vanishes as Parareal
 // converges, instead of with |U|, which does not.
 template <typename SG>
-__global__ void __launch_bounds__(256) k_correction_delta(int ni, int nj, int nk, View gd, View fine, View dst,
-                                                          View dnext, double* partials)
+__device__ __forceinline__ void correction_delta_body(int ni, int nj, int nk, const View& gd, const View& fine,
+                                                      const View& dst, const View& dnext, double* partials, int b,
+                                                      int nb)
 {
 #pragma clang fp contract(off)
     const long long n = (long long)ni * nj * nk;
     double d2 = 0., q2 = 0.;
-    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long long)gridDim.x * 256) {
+    for (long long p = (long long)b * 256 + threadIdx.x; p < n; p += (long long)nb * 256) {
         const int i = (int)(p % ni), j = (int)((p / ni) % nj), k = (int)(p / ((long long)ni * nj));
         const double g = gd.p ? (double)Store<SG>::load((const SG*)gd.p + i + j * gd.sj + k * gd.sk) : 0.;
-        const double b = ((const double*)fine.p)[i + j * fine.sj + k * fine.sk];
-        const double q = b + g;
+        const double bf = ((const double*)fine.p)[i + j * fine.sj + k * fine.sk];
+        const double q = bf + g;
         double* d = (double*)dst.p + i + j * dst.sj + k * dst.sk;
         const double diff = q - *d;
         *d = q;
@@ -149,7 +151,25 @@ __global__ void __launch_bounds__(256) k_correction_delta(int ni, int nj, int nk
     }
     double v[2] = {d2, q2};
     const bool mx[2] = {false, false};
-    block_reduce<2>(v, mx, partials);
+    block_reduce<2>(v, mx, partials, b);
+}
+
+template <typename SG>
+__global__ void __launch_bounds__(256) k_correction_delta(int ni, int nj, int nk, View gd, View fine, View dst,
+                                                          View dnext, double* partials)
+{
+    correction_delta_body<SG>(ni, nj, nk, gd, fine, dst, dnext, partials, blockIdx.x, gridDim.x);
+}
+
+// The same for several slices of independent problems in one launch: record e owns blocks
+// [e*nb, (e+1)*nb).  Records are read through the constant address space (scalar loads).
+template <typename SG>
+__global__ void __launch_bounds__(256) k_correction_delta_grouped(const CorrRecord* recs, int nb)
+{
+    typedef __attribute__((address_space(4))) const CorrRecord CRec;
+    const int e = blockIdx.x / nb;
+    const CorrRecord& r = *(const CorrRecord*)((CRec*)recs + e);
+    correction_delta_body<SG>(r.ni, r.nj, r.nk, r.gd, r.fine, r.dst, r.dnext, r.partials, blockIdx.x % nb, nb);
 }
 
 __global__ void __launch_bounds__(256) k_norms(int ni, int nj, int nk, View a, View b, double* partials)
@@ -291,6 +311,20 @@ int launch_correction_delta(hipStream_t s, int ni, int nj, int nk, int g_dtype, 
     else
         hipLaunchKernelGGL(k_correction_delta<bf16>, dim3(nblocks), dim3(256), 0, s, ni, nj, nk, gd, fine, dst,
                            dnext, partials);
+    SP_HIP(hipGetLastError());
+    return SP_OK;
+}
+
+int launch_correction_delta_grouped(hipStream_t s, int g_dtype, const CorrRecord* recs, int n, int nb)
+{
+    if (n <= 0) return SP_OK;
+    const dim3 grid((unsigned)(n * nb));
+    if (g_dtype == SP_F64)
+        hipLaunchKernelGGL(k_correction_delta_grouped<double>, grid, dim3(256), 0, s, recs, nb);
+    else if (g_dtype == SP_F32)
+        hipLaunchKernelGGL(k_correction_delta_grouped<float>, grid, dim3(256), 0, s, recs, nb);
+    else
+        hipLaunchKernelGGL(k_correction_delta_grouped<bf16>, grid, dim3(256), 0, s, recs, nb);
     SP_HIP(hipGetLastError());
     return SP_OK;
 }

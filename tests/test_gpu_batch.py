@@ -1,0 +1,118 @@
+"""Batches of independent Parareal problems (sp_batch_*, stencilparareal_amd.Batch).
+
+* A batch of small problems with FP64 G against the oracle's Parareal (the reference's
+  algorithm), problem by problem: the grouped launches and the difference form of the
+  correction give each problem the reference's iterate to 1e-12.
+* The same with FP32 G against single-problem GPU Parareal runs (difference form).
+* Problems of the uneven-batch configuration's sizes (64^3 .. 160^3, 8 .. 24 slices,
+  BF16-storage G): every problem converges to its own serial FP64 fine solution.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle as O  # noqa: E402
+from stencilparareal_amd.config import RuntimeConfiguration, uneven_batch  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def sp():
+    import stencilparareal_amd as spm
+    from stencilparareal_amd import field
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    field.Context.default(0)
+    return spm
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+SMALL = [  # n, slices, nu, c, T, steps_fine, steps_coarse
+    (8, 2, 0.03, (1.0, -0.5, 0.25), 0.01, 40, 4),
+    (12, 3, 0.01, (-0.7, 0.2, 0.9), 0.02, 66, 9),
+    (16, 4, 0.02, (0.3, 0.8, -0.6), 0.01, 64, 8),
+    (10, 1, 0.05, (0.5, 0.5, 0.5), 0.005, 20, 2),
+]
+
+
+def small_problems(sp):
+    return [sp.Problem(n, S, nu, c, T, sf, sc) for n, S, nu, c, T, sf, sc in SMALL]
+
+
+def gauss(n, i):
+    return O.fill_gaussians(n, [0.3 + 0.1 * i, 0.7], [0.4, 0.2], [0.6, 0.5 + 0.05 * i], [0.1, 0.07], [1.0, 0.6])
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_batch_fp64_coarse_matches_oracle_per_problem(sp, graphs):
+    K = 3
+    probs = small_problems(sp)
+    b = sp.Batch(probs, kmax=K, coarse_dtype="f64", use_graphs=graphs)
+    u0, q = b.fields("u"), b.fields("q")
+    for i, (f, p) in enumerate(zip(u0, probs)):
+        f.set(gauss(p.n, i))
+    rep = b.run(u0, q)
+    assert rep.iterations == K
+    for i, p in enumerate(probs):
+        conf = RuntimeConfiguration(gridSize=p.n, endTime=p.end_time, timeStepsFine=p.steps_fine,
+                                    timeStepsCoarse=p.steps_coarse, nu0=p.nu, nufreq=0.0, cx=p.c[0], cy=p.c[1],
+                                    cz=p.c[2], kmax=K, async_=False, timeSlices=p.slices)
+        oc = O.OConvection(p.n, p.n, p.n, 1.0 / p.n, p.nu, 0.0, *p.c)
+        out, _ = oc.parareal(gauss(p.n, i), p.slices, K, p.end_time, conf.dtFine(),
+                             conf.timeStepsFinePerTimeSlice(), conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(),
+                             "rk4", keep_iters=False)
+        e = relerr(q[i].get(), out)
+        assert e <= 1e-12, (i, e)
+        info = b.info(i)
+        assert info.steps_fine_per_slice == conf.timeStepsFinePerTimeSlice()
+        assert len(b.increments(i)) == K
+
+
+def test_batch_fp32_coarse_matches_single_problem_runs(sp):
+    K = 4
+    probs = small_problems(sp)
+    b = sp.Batch(probs, kmax=K, coarse_dtype="f32", use_graphs=True)
+    u0, q = b.fields("u"), b.fields("q")
+    for i, (f, p) in enumerate(zip(u0, probs)):
+        f.set(gauss(p.n, i))
+    b.run(u0, q)
+    for i, p in enumerate(probs):
+        conf = RuntimeConfiguration(gridSize=p.n, endTime=p.end_time, timeStepsFine=p.steps_fine,
+                                    timeStepsCoarse=p.steps_coarse, nu0=p.nu, nufreq=0.0, cx=p.c[0], cy=p.c[1],
+                                    cz=p.c[2], kmax=K, async_=False, timeSlices=p.slices)
+        conv = sp.Convection(p.n, p.n, p.n, 1.0 / p.n, p.nu, 0.0, *p.c)
+        sol = conv.field()
+        pr = sp.Parareal(conv, u0[i], sol, conf, coarse="rk4", coarse_dtype="f32", correction="delta")
+        pr.DoParallel()
+        e = relerr(q[i].get(), sol.get())
+        assert e <= 1e-12, (i, e)
+        del pr
+
+
+def test_batch_uneven_sizes_converge_to_serial_fine(sp):
+    # six problems drawn as in the uneven-batch configuration (its sizes, slices, nu, c)
+    ups = uneven_batch(count=6, seed=3)
+    probs = [sp.Problem(u.n, u.slices, u.nu, u.c, u.end_time, u.steps_fine, u.steps_coarse) for u in ups]
+    K = 6
+    b = sp.Batch(probs, kmax=K, coarse_dtype="bf16", tol=1e-10, use_graphs=True)
+    u0, q = b.fields("u"), b.fields("q")
+    for f, u in zip(u0, ups):
+        g = u.gauss
+        sp.fill_gaussians(f, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    rep = b.run(u0, q)
+    print("iterations", rep.iterations, "max increments", ["%.2e" % x for x in rep.increment[:rep.iterations]],
+          "GLUP", rep.lattice_updates / 1e9, "s", rep.t_total)
+    ref = b.fields("serial")
+    for i, u in enumerate(ups):
+        b.serial(i, u0[i], ref[i])
+        e = sp.rel_l2(q[i], ref[i])
+        inc = b.increments(i)
+        print(f"problem {i}: n={u.n} S={u.slices} nu={u.nu:.2e} rel L2 {e:.2e} increments",
+              ["%.1e" % x for x in inc])
+        assert e <= 1e-8, (i, e)
+        assert all(y < x or y <= 1e-13 for x, y in zip(inc, inc[1:]))
+    assert rep.iterations <= K
