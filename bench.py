@@ -57,9 +57,9 @@ def cpu_baseline(w, budget_s=12.0):
 
 
 def measure_batched_step(conv, init, conf, S, reps=50, copy_check=False):
-    """Average duration of one batched fused RK4 step over S slices (sp_do_rk4_batched,
-    steps = 1), timed with HIP events on the stream the library launches on (torch's
-    current stream, which the context follows).  Returns (ms, algorithmic bytes)."""
+    """Average duration of one batched fused RK4 step over S slices (sp_do_rk4_batched),
+    timed with HIP events on the stream the library launches on (torch's current stream,
+    which the context follows).  Returns (ms, algorithmic bytes)."""
     import torch
     from stencilparareal_amd import _lib as L
     N = init.layout.ni
@@ -72,16 +72,16 @@ def measure_batched_step(conv, init, conf, S, reps=50, copy_check=False):
     conv.ctx.sync_stream()
     dt_f = conf.dtFine()
 
-    def step(src, dst):
+    def steps(src, dst, k):
+        # one library call, k launches queued back to back (ping-pong with the library's
+        # batch scratch), so the events see kernel time rather than host-side call gaps
         L.call("sp_do_rk4_batched", conv.h, S, C_ptr(lay), ctypes_vp(src), C_ptr(lay), ctypes_vp(dst), n, 0.0,
-               dt_f, 1)
+               dt_f, k)
 
-    for _ in range(3):
-        step(bin_, bout)
+    steps(bin_, bout, 3)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record()
-    for r in range(reps):
-        step(*((bin_, bout) if r % 2 == 0 else (bout, bin_)))
+    steps(bin_, bout, reps)
     ev1.record()
     ev1.synchronize()
     k_ms = ev0.elapsed_time(ev1) / reps
@@ -119,12 +119,72 @@ def roofline_only(a):
                                 nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=w.kmax, async_=False,
                                 timeSlices=S)
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
+    conv.autotune(S)  # the tile the benchmark's Parareal run picks
     init = conv.field("qinitial")
     g = w.gauss
     sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
     k_ms, alg = measure_batched_step(conv, init, conf, S, reps=a.reps, copy_check=True)
     print(json.dumps({"kernel_avg_ms": round(k_ms, 4), "bytes_per_launch": alg, "tile": conv.get_tile(S, True),
                       "copy_bytes_read": alg / 2, "copy_bytes_written": alg / 2, "reps": a.reps}), flush=True)
+
+
+def uneven(a):
+    """The uneven batch: 24 independent problems (64^3 .. 160^3, 8 .. 24 slices, nu and c
+    drawn from seed 0) in one Parareal batch, F in FP64, G in FP32 (difference-form
+    correction), tiles autotuned per grid size, K = 5.  value = lattice updates of the
+    solves (fine + coarse) per second; per-problem error against its serial FP64 fine
+    solution on the same GPU."""
+    import torch
+    import stencilparareal_amd as sp
+    from stencilparareal_amd.config import uneven_batch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    ups = uneven_batch(24, seed=0)
+    probs = [sp.Problem(u.n, u.slices, u.nu, u.c, u.end_time, u.steps_fine, u.steps_coarse) for u in ups]
+    K = 5
+    log("uneven batch: create + autotune")
+    b = sp.Batch(probs, kmax=K, coarse_dtype="f32", use_graphs=bool(a.graphs), autotune=True)
+    u0, q, ref = b.fields("u"), b.fields("q"), b.fields("serial")
+    for f, u in zip(u0, ups):
+        g = u.gauss
+        sp.fill_gaussians(f, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    log(f"warmup {a.warmup}")
+    for _ in range(a.warmup):
+        b.run(u0, q)
+    torch.cuda.synchronize()
+    log(f"timed {a.steps}")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rep = b.run(u0, q)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / a.steps * 1e3
+    per = []
+    t_serial = 0.0
+    for i, u in enumerate(ups):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        b.serial(i, u0[i], ref[i])
+        torch.cuda.synchronize()
+        t_serial += time.perf_counter() - t1
+        inf = b.info(i)
+        per.append({"n": u.n, "slices": u.slices, "nu": float(f"{u.nu:.4e}"),
+                    "rel_l2_err": float(f"{sp.rel_l2(q[i], ref[i]):.3e}"),
+                    "tile": f"{'pair' if inf.fine_variant == 1 else 'scalar'} {inf.fine_tx}x{inf.fine_ty}x"
+                            f"{inf.fine_nthreads} k{inf.fine_kchunk}"})
+    out = {
+        "metric": "Parareal wall-time vs serial-FP64-fine at \u22641e-6 rel L2; stencil GLUP/s + %HBM peak",
+        "value": round(rep.lattice_updates / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (4 seeded Gaussians per problem, SplitMix64 seed 0)",
+        "config": {"workload": "uneven batch: 24 independent problems, n in {64,96,128,160}, slices in {8,16,24}, "
+                               "nu log-uniform [1e-3,1e-2], c uniform [-1,1]^3, coarse RK4 dt = half its stability "
+                               "limit, fine dt = coarse/16, 2 coarse steps per slice, F FP64, G FP32, K=5",
+                   "problems": len(ups), "kmax": K, "parallelism": "problems-and-slices-as-batch"},
+        "iterations": rep.iterations, "max_increment_per_iter": [float(f"{x:.3e}") for x in rep.increment[:rep.iterations]],
+        "serial_fine_wall_s_all_problems": round(t_serial, 4), "per_problem": per,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -138,6 +198,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the batched RK4 step and a copy kernel of the same size (for profilers)")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel with --roofline-only")
+    ap.add_argument("--workload", choices=["parareal128", "uneven"], default="parareal128",
+                    help="parareal128: the 128^3 / 16-slice FP64 solve (default); uneven: the 24-problem batch")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the time slices are the batch "
@@ -146,6 +208,8 @@ def main():
         sys.exit(2)
     if a.roofline_only:
         return roofline_only(a)
+    if a.workload == "uneven":
+        return uneven(a)
 
     import torch
 
@@ -164,7 +228,9 @@ def main():
     init, sol, serial = conv.field("qinitial"), conv.field("q"), conv.field("qreference")
     g = w.gauss
     sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
-    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype="f64", use_graphs=bool(a.graphs))
+    log("create (autotunes the fused tiles for these sizes)")
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype="f64", use_graphs=bool(a.graphs),
+                     autotune=True)
 
     # serial FP64 fine solution on the GPU (also the error reference)
     log("serial fine")
@@ -202,9 +268,18 @@ def main():
         errs_l2.append(sp.rel_l2(pr.solution(), serial))
         errs_inf.append(sp.error_reference(pr.solution(), serial))
 
-    # roofline of the dominant kernel: one batched fused RK4 step over all 16 slices
+    # roofline of the dominant kernel, the batched fused RK4 step over all 16 slices: one
+    # more iteration of the same solve without a graph, its fine sweep (tsF launches back
+    # to back on the solve's own slice arrays) timed with HIP events on the launch stream
     log("roofline")
-    k_ms, alg_bytes = measure_batched_step(conv, init, conf, S, reps=50)
+    conf1 = RuntimeConfiguration(gridSize=N, endTime=w.T, timeStepsFine=w.stepsF, timeStepsCoarse=w.stepsG,
+                                 nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=1, async_=False, timeSlices=S)
+    sol1 = conv.field("q1")
+    pr1 = sp.Parareal(conv, init, sol1, conf1, coarse="rk4", coarse_dtype="f64", use_graphs=False)
+    rep1 = pr1.DoParallel()
+    k_ms = rep1.ms_fine_events[0] / tsF
+    alg_bytes = 16.0 * S * N ** 3  # FP64 state read once + written once per lattice update
+    del pr1, sol1
     achieved = alg_bytes / (k_ms / 1e3) / 1e9
     tile = conv.get_tile(S, True)
     traffic = committed_traffic()

@@ -211,6 +211,8 @@ typedef struct sp_parareal_report {
     double t_fine[SP_MAX_ITERS];       /* seconds per iteration: batched fine sweep */
     double t_coarse[SP_MAX_ITERS];     /* coarse chain incl. updates */
     double t_total;
+    double ms_fine_events[SP_MAX_ITERS]; /* ms: the fine sweep between HIP events on the
+                                            stream (iterations run without a graph; 0 with) */
 } sp_parareal_report;
 
 int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out);

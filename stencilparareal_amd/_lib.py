@@ -68,7 +68,8 @@ class PararealReport(C.Structure):
                 ("steps_coarse_per_slice", C.c_int), ("dt_fine", C.c_double),
                 ("dt_coarse", C.c_double), ("increment", C.c_double * SP_MAX_ITERS),
                 ("t_init", C.c_double), ("t_fine", C.c_double * SP_MAX_ITERS),
-                ("t_coarse", C.c_double * SP_MAX_ITERS), ("t_total", C.c_double)]
+                ("t_coarse", C.c_double * SP_MAX_ITERS), ("t_total", C.c_double),
+                ("ms_fine_events", C.c_double * SP_MAX_ITERS)]
 
 
 class Problem(C.Structure):

@@ -716,6 +716,8 @@ struct sp_parareal {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::vector<double> part_host;
+    hipEvent_t ev[2] = {nullptr, nullptr};  // around the fine sweep (iterations without a graph)
+    double fine_ms = 0.;
 };
 
 namespace {
@@ -1005,6 +1007,8 @@ int sp_parareal_destroy(sp_parareal* pr)
     (void)hipStreamSynchronize(pr->p->ctx->stream);
     if (pr->gexec) (void)hipGraphExecDestroy(pr->gexec);
     if (pr->graph) (void)hipGraphDestroy(pr->graph);
+    for (hipEvent_t e : pr->ev)
+        if (e) (void)hipEventDestroy(e);
     for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->coefF, &pr->coefG, &pr->part,
                    &pr->D})
         b->release();
@@ -1122,9 +1126,19 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
         SP_HIP(hipStreamSynchronize(s));
         if (tfine) *tfine = now() - a;
         if (tcoarse) *tcoarse = 0.;
+        pr->fine_ms = 0.;
     } else {
+        if (!pr->ev[0]) {
+            SP_HIP(hipEventCreate(&pr->ev[0]));
+            SP_HIP(hipEventCreate(&pr->ev[1]));
+        }
+        SP_HIP(hipEventRecord(pr->ev[0], s));
         SP_TRY(fine_sweep(pr));
+        SP_HIP(hipEventRecord(pr->ev[1], s));
         SP_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        SP_HIP(hipEventElapsedTime(&ms, pr->ev[0], pr->ev[1]));
+        pr->fine_ms = ms;
         const double b = now();
         SP_TRY((pr->cfg.delta_correction ? coarse_chain_delta(pr) : coarse_chain(pr)));
         SP_HIP(hipStreamSynchronize(s));
@@ -1192,6 +1206,7 @@ int sp_parareal_run(sp_parareal* pr, const sp_layout* l0, const void* u0, const 
             rep->increment[k] = inc;
             rep->t_fine[k] = tf;
             rep->t_coarse[k] = tc;
+            rep->ms_fine_events[k] = pr->fine_ms;
             rep->iterations = k + 1;
         }
         if (pr->cfg.tol > 0 && inc <= pr->cfg.tol) {
