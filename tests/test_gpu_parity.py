@@ -3,7 +3,7 @@
 Bars:
   * reference-order paths (staged RK4 sweeps, Euler, periodic fill, update): bit-for-bit;
   * fused RK4 kernel (nested-stage form, FMA): max |gpu - oracle| / max |oracle| <= 1e-12
-    (the north-star tolerance for the fine propagator), typically ~1e-15;
+    (the tolerance stated for the fine propagator), typically ~1e-15;
   * coarse propagator in f32 / bf16 storage: checked against the f64 oracle with the
     tolerance of its storage precision, and through Parareal's convergence to the f64
     serial fine solution.
