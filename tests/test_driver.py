@@ -116,6 +116,7 @@ def oracle_convection_error(N, nu0, nufreq, c, T, steps, order):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("order", [4, 1])
+@pytest.mark.filterwarnings("ignore:Duplicate variable name")  # two arrays named q, as the reference writes
 def test_convection_driver_matches_oracle(tmp_path, order):
     N, T, steps, nu0, nufreq, c = 20, 0.01, 40, 0.1, 30.0, (1.0, -0.5, 0.25)
     out = run([CONVECTION, "--gridSize", str(N), "--timeSteps", str(steps), "--endTime", str(T), "--nu0", str(nu0),
