@@ -94,10 +94,11 @@ def measure_batched_step(conv, init, conf, S, reps=50, copy_check=False):
     return k_ms, 16.0 * S * n  # FP64 state read once + written once per lattice update
 
 
-def committed_traffic():
-    """HBM bytes per launch of the batched step, from the committed PMC summary (two
-    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a plain copy kernel)."""
-    path = os.path.join(ROOT, "profiles", "pmc_rk4_fused_pair_16x128cubed_fp64.txt")
+def committed_traffic(variant):
+    """HBM bytes per launch of the batched step with the given fused-kernel variant, from
+    the committed PMC summary (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled
+    by a plain copy kernel); None when no summary exists for that variant."""
+    path = os.path.join(ROOT, "profiles", f"pmc_rk4_fused_{variant}_16x128cubed_fp64.txt")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -119,7 +120,12 @@ def roofline_only(a):
                                 nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=w.kmax, async_=False,
                                 timeSlices=S)
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
-    conv.autotune(S)  # the tile the benchmark's Parareal run picks
+    if a.variant == "auto":
+        conv.autotune(S)  # the tile the benchmark's Parareal run picks
+    else:
+        # a fixed kernel: under counter collection the autotuner's timings are not the
+        # kernels' own, so profiles name the kernel they measure
+        conv.set_tile(32, 32, 512, 0, variant=a.variant)
     init = conv.field("qinitial")
     g = w.gauss
     sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
@@ -168,7 +174,7 @@ def uneven(a):
         inf = b.info(i)
         per.append({"n": u.n, "slices": u.slices, "nu": float(f"{u.nu:.4e}"),
                     "rel_l2_err": float(f"{sp.rel_l2(q[i], ref[i]):.3e}"),
-                    "tile": f"{'pair' if inf.fine_variant == 1 else 'scalar'} {inf.fine_tx}x{inf.fine_ty}x"
+                    "tile": f"{ {1: 'pair', 2: 'quad'}.get(inf.fine_variant, 'scalar')} {inf.fine_tx}x{inf.fine_ty}x"
                             f"{inf.fine_nthreads} k{inf.fine_kchunk}"})
     # the batch stopped by its increment test at 1e-6 (every problem's increment below it)
     log("uneven batch to tol 1e-6")
@@ -212,6 +218,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the batched RK4 step and a copy kernel of the same size (for profilers)")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel with --roofline-only")
+    ap.add_argument("--variant", choices=["auto", "quad", "pair"], default="auto",
+                    help="fused kernel for --roofline-only: autotuned (default) or a fixed 32x32x512 variant")
     ap.add_argument("--workload", choices=["parareal128", "uneven"], default="parareal128",
                     help="parareal128: the 128^3 / 16-slice FP64 solve (default); uneven: the 24-problem batch")
     a = ap.parse_args()
@@ -296,7 +304,7 @@ def main():
     del pr1, sol1
     achieved = alg_bytes / (k_ms / 1e3) / 1e9
     tile = conv.get_tile(S, True)
-    traffic = committed_traffic()
+    traffic = committed_traffic(tile["variant"])
 
     # the metric's wall time: the same solve stopped by its own increment test at 1e-6
     # (untimed in `value`, which is the configured K = 5 workload)
@@ -365,8 +373,8 @@ def main():
                     "fine_batched_kernel_est": round(k_ms * tsF / 1e3, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_rk4_fused_{tile['variant']}<{tile['tx']},{tile['ty']},{tile['nthreads']}> "
-                               f"(FP64, {S} slices of {N}^3, kchunk {tile['kchunk']})",
+                     "kernel": f"k_rk4_fused_{tile['variant']} {tile['tx']}x{tile['ty']} tiles, "
+                               f"{tile['nthreads']} threads (FP64, {S} slices of {N}^3, kchunk {tile['kchunk']})",
                      "avg_launch_ms": round(k_ms, 4), "bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

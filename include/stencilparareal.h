@@ -51,6 +51,7 @@ extern "C" {
 #define SP_KERNEL_AUTO -1
 #define SP_KERNEL_SCALAR 0 /* one column per lane, any sizes and storage types */
 #define SP_KERNEL_PAIR 1   /* FP64, two x-adjacent columns per lane, 16-byte accesses */
+#define SP_KERNEL_QUAD 2   /* FP64, 2x2 columns per lane in stages 1-3 (32x32 tiles, 512 threads) */
 
 /*
  * Field layout.  Element (i,j,k) of the interior, 0 <= i < ni etc., lives at

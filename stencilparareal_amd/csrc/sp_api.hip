@@ -435,7 +435,7 @@ int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int 
         p->tile = FusedTile{0, 0, 0, kchunk > 0 ? kchunk : -1, SP_KERNEL_AUTO};
         return SP_OK;
     }
-    SP_CHECK(variant == SP_KERNEL_SCALAR || variant == SP_KERNEL_PAIR, SP_ERR_INVALID,
+    SP_CHECK(variant == SP_KERNEL_SCALAR || variant == SP_KERNEL_PAIR || variant == SP_KERNEL_QUAD, SP_ERR_INVALID,
              "sp_prop_set_tile: unknown variant");
     const FusedTile t{tx, ty, nthreads, kchunk, variant};
     SP_CHECK(fused_tile_supported(t, 1), SP_ERR_UNSUPPORTED, "sp_prop_set_tile: tile shape not compiled");
@@ -450,7 +450,7 @@ int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* 
     SP_CHECK(p, SP_ERR_INVALID, "sp_prop_get_tile: null propagator");
     SP_CHECK(nslices >= 1, SP_ERR_INVALID, "sp_prop_get_tile: nslices < 1");
     FusedTile t = pick_tile(p, nslices, f64 ? 1 : 0);
-    if (!f64 && t.variant == SP_KERNEL_PAIR) t = FusedTile{32, 16, 512, t.kchunk, SP_KERNEL_SCALAR};
+    if (!f64 && t.variant != SP_KERNEL_SCALAR) t = FusedTile{32, 16, 512, t.kchunk, SP_KERNEL_SCALAR};
     if (variant) *variant = t.variant;
     if (tx) *tx = t.tx;
     if (ty) *ty = t.ty;

@@ -105,7 +105,7 @@ size_t arena_put(sp_batch* b, const void* data, size_t bytes)
     return off;
 }
 
-// Groups entries by tile shape (pair kernels need even, aligned fields: other entries take
+// Groups entries by tile shape (pair / 2x2 kernels need even, aligned fields: other entries take
 // the scalar 32x16x512 kernel) and appends the launches' records; returns the group index.
 int add_group(sp_batch* b, std::vector<Entry> ents, int idt, int odt, std::map<std::string, int>& memo,
               const std::string& key)
@@ -114,7 +114,7 @@ int add_group(sp_batch* b, std::vector<Entry> ents, int idt, int odt, std::map<s
     if (hit != memo.end()) return hit->second;
     std::map<std::vector<int>, std::vector<Entry>> by_shape;
     for (Entry& e : ents) {
-        if (e.tile.variant == SP_KERNEL_PAIR &&
+        if (e.tile.variant != SP_KERNEL_SCALAR &&
             !(idt == SP_F64 && odt == SP_F64 && fused_record_pair_ok(e.st, e.in, e.out)))
             e.tile = FusedTile{32, 16, 512, e.tile.kchunk, SP_KERNEL_SCALAR};
         by_shape[{e.tile.variant, e.tile.tx, e.tile.ty, e.tile.nt}].push_back(e);

@@ -453,12 +453,13 @@ struct PairMap {
         by = g / VW;
         ax = 2 * min(g % VW, WP - 1);
     }
-    // its centre in the input plane (row stride WI), and its place in the output plane
+    // its centre in the input plane (row stride WIN), and its place in the output plane
+    template <int WIN = G::WI>
     __device__ static int centre(int g)
     {
         int ax, by;
         at(g, ax, by);
-        return (by + 2) * G::WI + ax + 2;
+        return (by + 2) * WIN + ax + 2;
     }
     __device__ static int out(int g)
     {
@@ -468,16 +469,14 @@ struct PairMap {
     }
 };
 
-template <int TX, int TY, int S, int NT, int M, int R>
+template <int TX, int TY, int S, int NT, int M, int R, int WI = Geo<TX, TY, S>::WI>
 __device__ __forceinline__ void pair_step(const double* __restrict__ P, const StarW<double>& w, double (&acc)[M][2][4],
                                           double (&done)[M][2], d2 (&cen)[M])
 {
-    using G = Geo<TX, TY, S>;
-    constexpr int WI = G::WI;
     constexpr int sm2 = (R + 2) & 3, sm1 = (R + 3) & 3, s0 = R & 3, sp1 = (R + 1) & 3;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const double* c = P + PairMap<TX, TY, S, NT>::centre((int)threadIdx.x + m * NT);
+        const double* c = P + PairMap<TX, TY, S, NT>::template centre<WI>((int)threadIdx.x + m * NT);
         // x group first (centres and x neighbours), then the y group: at most 4 pairs of
         // loaded values are live at a time
         const d2 C = *(const d2*)c, Lx = *(const d2*)(c - 2), Rx = *(const d2*)(c + 2);
@@ -745,6 +744,306 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
     }
 }
 
+// ---- FP64 2x2 variant (32x32 tiles, 512 threads) -----------------------------------------
+// Stages 1-3 give each lane a 2x2 block of columns (an x pair in two consecutive rows).  The
+// centres of one row are y neighbours of the other, so 10 ds_read_b128 per block and stage
+// give what the pair layout reads with 14; the points per lane (4 + 4 + 4 + 2) and with
+// them the register budget are the pair kernel's.  Stage 4 keeps the pair layout.
+// Bank conflicts: a lane row of VW blocks steps W 16-byte chunks per block row (two plane
+// rows of W/2 chunks), so a plane stride W = VW (mod 16) puts consecutive lanes on
+// consecutive chunks across row ends, for reads (input plane) and writes (output plane)
+// alike.  VW = 22 (stage 1's 22 blocks per row), plane strides 54 and 38 doubles.  In
+// stages 2 and 3 the lanes past a row's last block are spare: they read the input plane's
+// padding columns (so their groups keep the same chunk pattern) and write nothing.
+// q: stage 1 keeps it in registers, stages 2 and 3 load it from global memory at lags 5
+// and 8 (one plane ahead), stage 3 forwards it to stage 4 through the LDS plane Q4.
+struct QuadCfg {
+    static constexpr int TX = 32, TY = 32, NT = 512, VW = 22;
+    static constexpr int W0 = 54, W1 = 54, W2 = 54, W3 = 38;  // strides of P0..P3
+    static constexpr int WQ = 38, HQ = 34;                     // Q4: stage 4's region + spare rows
+    static constexpr int VW0 = 27;                             // lanes per row of P0's loader
+};
+
+template <int S>
+struct QuadMap {
+    using G = Geo<QuadCfg::TX, QuadCfg::TY, S>;
+    static constexpr int U = G::WO / 2;  // blocks per row = block rows
+    static constexpr int VW = QuadCfg::VW, SLOTS = VW * U;
+    static constexpr int WIN = S == 1 ? QuadCfg::W0 : S == 2 ? QuadCfg::W1 : QuadCfg::W2;
+    static constexpr int WOUT = S == 1 ? QuadCfg::W1 : S == 2 ? QuadCfg::W2 : QuadCfg::W3;
+    static_assert(G::WO == G::HOo && U <= VW && SLOTS <= QuadCfg::NT, "one pass of square blocks");
+    static_assert((WIN - VW) % 16 == 0 && (WOUT - VW) % 16 == 0, "conflict-free plane strides");
+    static_assert(WIN >= G::WI && WOUT >= G::WO, "strides cover the regions");
+    static_assert(WIN >= 2 * VW + 4, "spare lanes read inside the padded rows");
+    // block of slot g: column 2*ux and rows 2*uy, 2*uy + 1 of the output region; slots past
+    // the last one repeat it, ux >= U marks a spare lane
+    __device__ static void at(int g, int& ux, int& uy)
+    {
+        g = min(g, SLOTS - 1);
+        uy = g / VW;
+        ux = g % VW;
+    }
+    __device__ static bool live(int g)
+    {
+        int ux, uy;
+        at(g, ux, uy);
+        return ux < U;
+    }
+    __device__ static int centre(int g)  // first row's centre in the input plane
+    {
+        int ux, uy;
+        at(g, ux, uy);
+        return (2 * uy + 2) * WIN + 2 * ux + 2;
+    }
+    __device__ static int out(int g)
+    {
+        int ux, uy;
+        at(g, ux, uy);
+        return 2 * uy * WOUT + 2 * ux;
+    }
+};
+
+// One stage at one plane for a 2x2 block; point p = 2 f + e (row f, column e).
+template <int S, int R>
+__device__ __forceinline__ void quad_step(const double* __restrict__ P, const StarW<double>& w, double (&acc)[4][4],
+                                          double (&done)[4], d2 (&cen)[2])
+{
+    constexpr int W = QuadMap<S>::WIN;
+    constexpr int sm2 = (R + 2) & 3, sm1 = (R + 3) & 3, s0 = R & 3, sp1 = (R + 1) & 3;
+    const double* c = P + QuadMap<S>::centre((int)threadIdx.x);
+    double ax[4];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        // centres and x neighbours of row f, and the z sums
+        const d2 C = *(const d2*)(c + f * W), Lx = *(const d2*)(c + f * W - 2), Rx = *(const d2*)(c + f * W + 2);
+        cen[f] = C;
+        const double u[2] = {C.x, C.y};
+        const double xm2[2] = {Lx.x, Lx.y}, xm1[2] = {Lx.y, C.x}, xp1[2] = {C.y, Rx.x}, xp2[2] = {Rx.x, Rx.y};
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int p = 2 * f + e;
+            done[p] = fmaT(w.zp2, u[e], acc[p][sm2]);
+            acc[p][sm2] = w.zm2 * u[e];
+            acc[p][sm1] = fmaT(w.zp1, u[e], acc[p][sm1]);
+            acc[p][sp1] = fmaT(w.zm1, u[e], acc[p][sp1]);
+            double a = fmaT(w.c, u[e], acc[p][s0]);
+            a = fmaT(w.xm2, xm2[e], a);
+            a = fmaT(w.xm1, xm1[e], a);
+            a = fmaT(w.xp1, xp1[e], a);
+            ax[p] = fmaT(w.xp2, xp2[e], a);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    {
+        // y neighbours: rows y-2, y-1, y+2, y+3 and the two centre rows
+        const d2 Ym2 = *(const d2*)(c - 2 * W), Ym1 = *(const d2*)(c - W);
+        const d2 Yp2 = *(const d2*)(c + 2 * W), Yp3 = *(const d2*)(c + 3 * W);
+        const double r[6][2] = {{Ym2.x, Ym2.y}, {Ym1.x, Ym1.y}, {cen[0].x, cen[0].y},
+                                {cen[1].x, cen[1].y}, {Yp2.x, Yp2.y}, {Yp3.x, Yp3.y}};
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                // row y+f: neighbours at rows (y+f) -2, -1, +1, +2 = r[f], r[f+1], r[f+3], r[f+4]
+                double a = w.ym2 * r[f][e];
+                a = fmaT(w.ym1, r[f + 1][e], a);
+                a = fmaT(w.yp1, r[f + 3][e], a);
+                a = fmaT(w.yp2, r[f + 4][e], a);
+                acc[2 * f + e][s0] = ax[2 * f + e] + a;
+            }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Global byte offsets of the two rows of slot g's block for stage S (periodic images).
+template <int S>
+__device__ __forceinline__ void quad_offsets(int (&q)[2], int x0, int y0, int ni, int nj, int sj)
+{
+    using G = Geo<QuadCfg::TX, QuadCfg::TY, S>;
+    int ux, uy;
+    QuadMap<S>::at((int)threadIdx.x, ux, uy);
+    const int gx = pmod(x0 - G::HO + 2 * ux, ni);
+#pragma unroll
+    for (int f = 0; f < 2; ++f) q[f] = (gx + pmod(y0 - G::HO + 2 * uy + f, nj) * sj) * 8;
+}
+
+__device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
+{
+    constexpr int NT = QuadCfg::NT, TX = QuadCfg::TX, TY = QuadCfg::TY;
+    using G1 = Geo<TX, TY, 1>;
+    using G4 = Geo<TX, TY, 4>;
+    using PM4 = PairMap<TX, TY, 4, NT>;
+    static_assert(PM4::M == 1 && PM4::VW == 16, "stage 4: one pass, one row per 16-lane group");
+    constexpr int H0 = G1::HI;                         // rows of the q plane incl. halo
+    constexpr int SLOTS0 = QuadCfg::VW0 * H0;          // loader slots (cols >= WI/2 fill padding)
+    constexpr int M0 = (SLOTS0 + NT - 1) / NT;
+    static_assert(QuadCfg::VW0 * 2 <= QuadCfg::W0 && (QuadCfg::W0 / 2 - QuadCfg::VW0) % 16 == 0, "loader layout");
+    constexpr int W0 = QuadCfg::W0, W1 = QuadCfg::W1, W2 = QuadCfg::W2, W3 = QuadCfg::W3, WQ = QuadCfg::WQ;
+
+    __shared__ __attribute__((aligned(16))) double P0[2][H0 * W0];
+    __shared__ __attribute__((aligned(16))) double P1[2][Geo<TX, TY, 2>::HI * W1];
+    __shared__ __attribute__((aligned(16))) double P2[2][Geo<TX, TY, 3>::HI * W2];
+    __shared__ __attribute__((aligned(16))) double P3[2][G4::HI * W3];
+    __shared__ __attribute__((aligned(16))) double Q4[2][QuadCfg::HQ * WQ];
+
+    const int tid = threadIdx.x;
+    const int ni = a.ni, nj = a.nj, nk = a.nk;
+    const int tile = bid % a.tiles;
+    const int chunk = (bid / a.tiles) % a.chunks;
+    const int slice = bid / (a.tiles * a.chunks);
+    const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
+    const int z0 = chunk * a.kchunk;
+    const int zend = min(z0 + a.kchunk, nk);
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc((const double*)a.in.p + slice * a.in.slice, a.in_bytes);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc((double*)a.out.p + slice * a.out.slice, a.out_bytes);
+    const int iskb = (int)a.in.sk * 8, oskb = (int)a.out.sk * 8;
+    const int isj = (int)a.in.sj, osj = (int)a.out.sj;
+    const StarW<double> w = a.cdev ? load_w<double>(a.cdev[slice]) : load_w<double>(a.c);
+
+    // q plane loader: VW0 lanes per row; lanes past the row's 24 pairs load the periodic
+    // continuation into the row's padding (never read), which keeps the writes conflict-free
+    int off0[M0], dst0[M0];
+#pragma unroll
+    for (int m = 0; m < M0; ++m) {
+        const int g = min(tid + m * NT, SLOTS0 - 1);
+        const int col = g % QuadCfg::VW0, row = g / QuadCfg::VW0;
+        off0[m] = (pmod(x0 - 8 + 2 * col, ni) + pmod(y0 - 8 + row, nj) * isj) * 8;
+        dst0[m] = row * W0 + 2 * col;
+    }
+    int q2[2], q3[2];
+    quad_offsets<2>(q2, x0, y0, ni, nj, isj);
+    quad_offsets<3>(q3, x0, y0, ni, nj, isj);
+    const int o1 = QuadMap<1>::out(tid), o2 = QuadMap<2>::out(tid), o3 = QuadMap<3>::out(tid);
+    static_assert(QuadMap<1>::U == QuadCfg::VW, "stage 1 has no spare lanes");
+    const bool live2 = QuadMap<2>::live(tid), live3 = QuadMap<3>::live(tid);
+    int wq;  // stage 3's block in Q4 (stage 4's region); blocks outside it go to spare cells
+    {
+        int ux, uy;
+        QuadMap<3>::at(tid, ux, uy);
+        int qx = 2 * ux - 2, qy = 2 * uy - 2;
+        if (qx < 0) qx = G4::WO + 2;  // spare columns G4::WO .. WQ-1
+        if (qy < 0) qy = G4::HOo;     // spare rows G4::HOo .. HQ-1
+        wq = qy * WQ + qx;
+    }
+    int o4, r4;
+    {
+        int ax, by;
+        PM4::at(tid, ax, by);
+        o4 = (pmod(x0 + ax, ni) + pmod(y0 + by, nj) * osj) * 8;  // periodic image for partial tiles
+        r4 = by * WQ + ax;
+    }
+
+    double acc1[4][4], acc2[4][4], acc3[4][4], acc4[1][2][4];
+    double d1[4], d2v[4], d3[4], d4[1][2];
+    d2 c1[2], c2[2], c3[2], c4[1];
+    d2 qv2[2], qv3[2];
+    d2 r1a[2], r1b[2], r4a, r4b;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc1[p][j] = acc2[p][j] = acc3[p][j] = 0.;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) acc4[0][e][0] = acc4[0][e][1] = acc4[0][e][2] = acc4[0][e][3] = 0.;
+    r1a[0] = r1a[1] = r1b[0] = r1b[1] = r4a = r4b = d2{0., 0.};
+
+    // Skewed schedule of the pair kernel: iteration t, stage 1: q t -> v1 t-2; stage 2:
+    // v1 t-3 -> v2 t-5; stage 3: v2 t-6 -> v3 t-8; stage 4: v3 t-9 -> q' t-11.
+    const int tq0 = z0 - 8, tq1 = zend + 8;
+    const int tend = zend + 11;
+    {
+        const int zo = wrapz(tq0, nk) * iskb;
+#pragma unroll
+        for (int m = 0; m < M0; ++m) *(d2*)&P0[tq0 & 1][dst0[m]] = ld2(rin, off0[m], zo);
+        const int z2 = wrapz(z0 - 8, nk) * iskb, z3 = wrapz(z0 - 6, nk) * iskb;
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            qv2[f] = ld2(rin, q2[f], z2);
+            qv3[f] = ld2(rin, q3[f], z3);
+        }
+    }
+    d2 pre[M0];
+    if (tq0 + 1 < tq1) {
+        const int zo = wrapz(tq0 + 1, nk) * iskb;
+#pragma unroll
+        for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+    }
+    __syncthreads();
+
+    auto plane = [&](auto Rc, int t) {
+        constexpr int R = decltype(Rc)::value;
+        const int cur = t & 1, prv = cur ^ 1;
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + 1 < tq1) {
+#pragma unroll
+            for (int m = 0; m < M0; ++m) *(d2*)&P0[prv][dst0[m]] = pre[m];
+            if (t + 2 < tq1) {
+                const int zo = wrapz(t + 2, nk) * iskb;
+#pragma unroll
+                for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (t < tq1) {
+            // centres of plane t are q(t); r1b = q(t-2) is the update's q
+            quad_step<1, R & 3>(P0[cur], w, acc1, d1, c1);
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                *(d2*)&P1[cur][o1 + f * W1] =
+                    d2{fmaT(w.f1, d1[2 * f], r1b[f].x), fmaT(w.f1, d1[2 * f + 1], r1b[f].y)};
+                r1b[f] = r1a[f];
+                r1a[f] = c1[f];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (t >= tq0 + 5 && t < tq1 + 1) {
+            quad_step<2, (R + 1) & 3>(P1[prv], w, acc2, d2v, c2);
+            if (live2) {
+#pragma unroll
+                for (int f = 0; f < 2; ++f)
+                    *(d2*)&P2[cur][o2 + f * W2] =
+                        d2{fmaT(w.f2, d2v[2 * f], qv2[f].x), fmaT(w.f2, d2v[2 * f + 1], qv2[f].y)};
+            }
+            const int zo = wrapz(t - 4, nk) * iskb;  // q(t-4): the next plane's q
+#pragma unroll
+            for (int f = 0; f < 2; ++f) qv2[f] = ld2(rin, q2[f], zo);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (t >= tq0 + 10 && t < tq1 + 2) {
+            quad_step<3, (R + 2) & 3>(P2[prv], w, acc3, d3, c3);
+            if (live3) {
+#pragma unroll
+                for (int f = 0; f < 2; ++f) {
+                    *(d2*)&P3[cur][o3 + f * W3] =
+                        d2{fmaT(w.f3, d3[2 * f], qv3[f].x), fmaT(w.f3, d3[2 * f + 1], qv3[f].y)};
+                    *(d2*)&Q4[cur][wq + f * WQ] = qv3[f];  // q(t-8) for stage 4
+                }
+            }
+            const int zo = wrapz(t - 7, nk) * iskb;  // q(t-7): the next plane's q
+#pragma unroll
+            for (int f = 0; f < 2; ++f) qv3[f] = ld2(rin, q3[f], zo);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (t >= tq0 + 15) {
+            pair_step<TX, TY, 4, NT, 1, (R + 3) & 3, W3>(P3[prv], w, acc4, d4, c4);
+            const int zp = t - 11;
+            if (zp >= z0 && zp < zend)
+                st2(rout, o4, zp * oskb, d2{fmaT(w.f4, d4[0][0], r4b.x), fmaT(w.f4, d4[0][1], r4b.y)});
+            r4b = r4a;
+            r4a = *(const d2*)&Q4[prv][r4];  // q(t-9)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    for (int t = tq0; t < tend; t += 4) {
+        plane(std::integral_constant<int, 0>{}, t);
+        if (t + 1 < tend) plane(std::integral_constant<int, 1>{}, t + 1);
+        if (t + 2 < tend) plane(std::integral_constant<int, 2>{}, t + 2);
+        if (t + 3 < tend) plane(std::integral_constant<int, 3>{}, t + 3);
+    }
+}
+
 // One launch over one batch of slices.
 template <typename SI, typename SO, typename T, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
@@ -756,6 +1055,11 @@ template <int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
 {
     rk4_pair_body<TX, TY, NT>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+
+__global__ void __launch_bounds__(QuadCfg::NT) k_rk4_fused_quad(FusedArgs a)
+{
+    rk4_quad_body(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // One launch over several independent batches (problems of different sizes, steps and
@@ -788,6 +1092,14 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair_grouped(GroupArgs g)
     const int e = group_entry(g, bid);
     ConstArgs* rec = (ConstArgs*)g.args + e;
     rk4_pair_body<TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
+}
+
+__global__ void __launch_bounds__(QuadCfg::NT) k_rk4_fused_quad_grouped(GroupArgs g)
+{
+    const int bid = xcd_block(blockIdx.x, gridDim.x);
+    const int e = group_entry(g, bid);
+    ConstArgs* rec = (ConstArgs*)g.args + e;
+    rk4_quad_body(*(const FusedArgs*)rec, bid - g.first[e]);
 }
 
 }  // namespace
@@ -886,6 +1198,17 @@ int launch_pair(hipStream_t s, const FusedStep& st, const FusedTile& tile, View 
     return fail(SP_ERR_UNSUPPORTED, "fused RK4 (pair): unsupported tile shape");
 }
 
+int launch_quad(hipStream_t s, const FusedStep& st, const FusedTile& tile, View in, View out)
+{
+    SP_CHECK(tile.tx == QuadCfg::TX && tile.ty == QuadCfg::TY && tile.nt == QuadCfg::NT, SP_ERR_UNSUPPORTED,
+             "fused RK4 (2x2): unsupported tile shape");
+    FusedArgs a;
+    SP_TRY(fill_args(a, st, tile, in, out, 8, 8));
+    hipLaunchKernelGGL(k_rk4_fused_quad, dim3(nblocks_of(a, st.nslices)), dim3(QuadCfg::NT), 0, s, a);
+    SP_HIP(hipGetLastError());
+    return SP_OK;
+}
+
 // 16-byte pair access needs even x extent, even strides and 16-byte aligned origins.
 bool pair_ok(const FusedStep& st, View in, View out)
 {
@@ -935,6 +1258,14 @@ int launch_rk4_grouped(hipStream_t s, const FusedTile& tile, int in_dtype, int o
 {
     SP_CHECK(n >= 1 && nblocks >= 1 && nblocks <= 0x7fffffffLL, SP_ERR_INVALID, "grouped fused RK4: empty or too large");
     const GroupArgs g{(const FusedArgs*)records, first, n};
+    if (tile.variant == SP_KERNEL_QUAD) {
+        SP_CHECK(in_dtype == SP_F64 && out_dtype == SP_F64, SP_ERR_UNSUPPORTED, "grouped 2x2 kernel is FP64 only");
+        SP_CHECK(tile.tx == QuadCfg::TX && tile.ty == QuadCfg::TY && tile.nt == QuadCfg::NT, SP_ERR_UNSUPPORTED,
+                 "grouped fused RK4 (2x2): unsupported tile shape");
+        hipLaunchKernelGGL(k_rk4_fused_quad_grouped, dim3((unsigned)nblocks), dim3(QuadCfg::NT), 0, s, g);
+        SP_HIP(hipGetLastError());
+        return SP_OK;
+    }
     if (tile.variant == SP_KERNEL_PAIR) {
         SP_CHECK(in_dtype == SP_F64 && out_dtype == SP_F64, SP_ERR_UNSUPPORTED, "grouped pair kernel is FP64 only");
         const dim3 grid((unsigned)nblocks);
@@ -962,6 +1293,7 @@ bool fused_tile_supported(const FusedTile& t, int /*compute_f64*/)
 {
     static const int scalar[][3] = {{32, 32, 1024}, {32, 24, 1024}, {24, 24, 1024}, {32, 16, 512}, {16, 16, 256}};
     static const int pair[][3] = {{32, 32, 512}, {32, 16, 512}, {16, 16, 256}};
+    if (t.variant == SP_KERNEL_QUAD) return t.tx == QuadCfg::TX && t.ty == QuadCfg::TY && t.nt == QuadCfg::NT;
     if (t.variant == SP_KERNEL_PAIR) {
         for (const auto& sh : pair)
             if (t.tx == sh[0] && t.ty == sh[1] && t.nt == sh[2]) return true;
@@ -974,17 +1306,18 @@ bool fused_tile_supported(const FusedTile& t, int /*compute_f64*/)
 
 FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64)
 {
-    // Measured on MI355X (16 x 128^3 and 1 x 128^3 FP64): the spill-free x-pair kernel with
-    // 32x32 tiles of 512 threads is fastest once the tiles alone give one block per CU;
-    // below that, 32x16 tiles chunked in k fill the chip better.  FP32 / BF16 storage runs
-    // the scalar kernel.
+    // Measured on MI355X (FP64; 16 x 128^3, 8 x 160^3, 24 x 96^3, 1 x 128^3): the 2x2
+    // kernel (32x32 tiles of 512 threads) is fastest once the tiles alone give one block
+    // per CU; below that, x-pair 32x16 tiles chunked in k fill the chip better.  FP32 / BF16
+    // storage runs the scalar kernel.
     const long long want = 256;  // one block per CU
-    FusedTile t{32, 32, 512, 0, compute_f64 ? SP_KERNEL_PAIR : SP_KERNEL_SCALAR};
+    FusedTile t{32, 32, 512, 0, compute_f64 ? SP_KERNEL_QUAD : SP_KERNEL_SCALAR};
     if (!compute_f64) t.nt = 1024;
     long long blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
     if (blocks < want) {
         t.ty = 16;
         t.nt = 512;
+        if (compute_f64) t.variant = SP_KERNEL_PAIR;
         blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
     }
     int chunks = (int)((want + blocks - 1) / blocks);
@@ -1000,8 +1333,10 @@ void fused_candidates(int ni, int nj, int nk, int nslices, int compute_f64, std:
     static const int scalar[][3] = {{32, 32, 1024}, {32, 24, 1024}, {24, 24, 1024}, {32, 16, 512}, {16, 16, 256}};
     static const int pair[][3] = {{32, 32, 512}, {32, 16, 512}, {16, 16, 256}};
     std::vector<FusedTile> shapes;
-    if (compute_f64 && ni % 2 == 0)
+    if (compute_f64 && ni % 2 == 0) {
+        shapes.push_back(FusedTile{QuadCfg::TX, QuadCfg::TY, QuadCfg::NT, 0, SP_KERNEL_QUAD});
         for (const auto& sh : pair) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_PAIR});
+    }
     for (const auto& sh : scalar) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_SCALAR});
     out.clear();
     for (const FusedTile& t : shapes) {
@@ -1028,8 +1363,9 @@ int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile_i
     if (st.nk < kFusedMinK)
         return fail(SP_ERR_UNSUPPORTED, "fused RK4: ksize " + std::to_string(st.nk) + " < " +
                                             std::to_string(kFusedMinK) + " (use the staged path)");
+    if (f64 && tile.variant == SP_KERNEL_QUAD && pair_ok(st, in, out)) return launch_quad(s, st, tile, in, out);
     if (f64 && tile.variant == SP_KERNEL_PAIR && pair_ok(st, in, out)) return launch_pair(s, st, tile, in, out);
-    if (tile.variant == SP_KERNEL_PAIR) {
+    if (tile.variant == SP_KERNEL_PAIR || tile.variant == SP_KERNEL_QUAD) {
         // odd sizes, unaligned views or reduced-precision storage: the scalar kernel with
         // its fastest FP64 shape and the caller's k chunk
         tile.tx = 32;

@@ -136,7 +136,7 @@ int launch_euler_fused(hipStream_t s, const StagedParams& P, View qin, View qout
 // Fused RK4 step over `nslices` batched slices.  Types by dtype codes.
 struct FusedTile {
     int tx, ty, nt, kchunk;
-    int variant;  // SP_KERNEL_SCALAR or SP_KERNEL_PAIR
+    int variant;  // SP_KERNEL_SCALAR, SP_KERNEL_PAIR or SP_KERNEL_QUAD
 };
 struct FusedStep {
     int ni, nj, nk;

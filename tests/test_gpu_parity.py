@@ -92,7 +92,7 @@ def test_euler_bitwise(sp, upwind, cvec):
 
 
 TILES = [("scalar", 32, 32, 1024), ("scalar", 24, 24, 1024), ("scalar", 32, 16, 512), ("scalar", 16, 16, 256),
-         ("pair", 32, 32, 512), ("pair", 32, 16, 512), ("pair", 16, 16, 256)]
+         ("pair", 32, 32, 512), ("pair", 32, 16, 512), ("pair", 16, 16, 256), ("quad", 32, 32, 512)]
 
 
 @pytest.mark.parametrize("N", [8, 16, 20, 32, 48, 64])
@@ -109,7 +109,7 @@ def test_fused_rk4_vs_oracle(sp, N, tile):
     assert relerr(b.get(), ref) <= 1e-12
 
 
-@pytest.mark.parametrize("variant", ["scalar", "pair"])
+@pytest.mark.parametrize("variant", ["scalar", "pair", "quad"])
 @pytest.mark.parametrize("kchunk", [0, 7, 16])
 def test_fused_rk4_kchunks_and_nonsquare(sp, kchunk, variant):
     ni, nj, nk = 40, 24, 36
@@ -135,26 +135,29 @@ def test_fused_rk4_aliasing_and_zero_steps(sp, steps):
     assert relerr(a.get(), ref) <= 1e-13
 
 
-def test_pair_and_scalar_kernels_agree_bitwise(sp):
-    # both variants evaluate every point with the same operation sequence
+def test_pair_quad_and_scalar_kernels_agree_bitwise(sp):
+    # all variants evaluate every point with the same operation sequence
     N = 48
     q0 = rand_field((N, N, N), 9)
     outs = []
-    for variant, tile in (("scalar", (32, 32, 1024)), ("pair", (32, 32, 512)), ("pair", (32, 16, 512))):
+    for variant, tile in (("scalar", (32, 32, 1024)), ("pair", (32, 32, 512)), ("pair", (32, 16, 512)),
+                          ("quad", (32, 32, 512))):
         conv = sp.Convection(N, N, N, 1. / N, 0.01, 0.0, *C3)
         conv.set_tile(*tile, kchunk=16, variant=variant)
         a, b = conv.field(), conv.field()
         a.set(q0)
         conv.DoRK4(a, b, 0.0, 1e-4, 3)
         outs.append(b.get())
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
 
 
-def test_odd_sizes_fall_back_to_scalar(sp):
+@pytest.mark.parametrize("variant", ["pair", "quad"])
+def test_odd_sizes_fall_back_to_scalar(sp, variant):
     ni, nj, nk = 21, 18, 17
     q0 = rand_field((nk, nj, ni), 6)
     conv = sp.Convection(ni, nj, nk, 1. / 21, 0.01, 0.0, *C3)
-    conv.set_tile(32, 32, 512, 0, variant="pair")
+    conv.set_tile(32, 32, 512, 0, variant=variant)
     a, b = conv.field(), conv.field()
     a.set(q0)
     conv.DoRK4(a, b, 0.0, 1e-4, 2)
@@ -188,7 +191,7 @@ def test_autotuned_tile_is_used_and_exact(sp, nslices, dtype):
     us = conv.autotune(nslices, dtype, dtype)
     assert 0 < us < 1e6
     t = conv.get_tile(nslices, dtype == "f64")
-    assert t["variant"] in ("scalar", "pair") and t["tx"] > 0 and t["kchunk"] >= 0
+    assert t["variant"] in ("scalar", "pair", "quad") and t["tx"] > 0 and t["kchunk"] >= 0
     if dtype == "f64":
         q0 = smooth_field(N, 11)
         a, b = conv.field(), conv.field()
