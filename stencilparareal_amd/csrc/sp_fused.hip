@@ -757,6 +757,8 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 // padding columns (so their groups keep the same chunk pattern) and write nothing.
 // q: stage 1 keeps it in registers, stages 2 and 3 load it from global memory at lags 5
 // and 8 (one plane ahead), stage 3 forwards it to stage 4 through the LDS plane Q4.
+// The q plane for P0 is loaded two planes ahead: at one plane (~1.9 us at 16 x 128^3) the
+// loads' HBM latency still showed, 4-9 % of the step.
 struct QuadCfg {
     static constexpr int TX = 32, TY = 32, NT = 512, VW = 22;
     static constexpr int W0 = 54, W1 = 54, W2 = 54, W3 = 38;  // strides of P0..P3
@@ -775,8 +777,8 @@ struct QuadMap {
     static_assert((WIN - VW) % 16 == 0 && (WOUT - VW) % 16 == 0, "conflict-free plane strides");
     static_assert(WIN >= G::WI && WOUT >= G::WO, "strides cover the regions");
     static_assert(WIN >= 2 * VW + 4, "spare lanes read inside the padded rows");
-    // block of slot g: column 2*ux and rows 2*uy, 2*uy + 1 of the output region; slots past
-    // the last one repeat it, ux >= U marks a spare lane
+    // block of slot g: column 2*ux and rows 2*uy, 2*uy + 1 of the output region; ux >= U
+    // marks a spare lane; lanes past the last slot repeat it (and write nothing)
     __device__ static void at(int g, int& ux, int& uy)
     {
         g = min(g, SLOTS - 1);
@@ -910,12 +912,17 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
         off0[m] = (pmod(x0 - 8 + 2 * col, ni) + pmod(y0 - 8 + row, nj) * isj) * 8;
         dst0[m] = row * W0 + 2 * col;
     }
+    const auto wr0 = [&](int m) { return m < M0 - 1 || tid + m * NT < SLOTS0; };  // loader tail: no write
     int q2[2], q3[2];
     quad_offsets<2>(q2, x0, y0, ni, nj, isj);
     quad_offsets<3>(q3, x0, y0, ni, nj, isj);
     const int o1 = QuadMap<1>::out(tid), o2 = QuadMap<2>::out(tid), o3 = QuadMap<3>::out(tid);
     static_assert(QuadMap<1>::U == QuadCfg::VW, "stage 1 has no spare lanes");
-    const bool live2 = QuadMap<2>::live(tid), live3 = QuadMap<3>::live(tid);
+    // lanes past a stage's last slot repeat it and write nothing (several lanes writing one
+    // LDS address serialise; reads of one address are a broadcast)
+    const bool live1 = tid < QuadMap<1>::SLOTS;
+    const bool live2 = tid < QuadMap<2>::SLOTS && QuadMap<2>::live(tid);
+    const bool live3 = tid < QuadMap<3>::SLOTS && QuadMap<3>::live(tid);
     int wq;  // stage 3's block in Q4 (stage 4's region); blocks outside it go to spare cells
     {
         int ux, uy;
@@ -953,7 +960,8 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
     {
         const int zo = wrapz(tq0, nk) * iskb;
 #pragma unroll
-        for (int m = 0; m < M0; ++m) *(d2*)&P0[tq0 & 1][dst0[m]] = ld2(rin, off0[m], zo);
+        for (int m = 0; m < M0; ++m)
+            if (wr0(m)) *(d2*)&P0[tq0 & 1][dst0[m]] = ld2(rin, off0[m], zo);
         const int z2 = wrapz(z0 - 8, nk) * iskb, z3 = wrapz(z0 - 6, nk) * iskb;
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
@@ -961,11 +969,16 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
             qv3[f] = ld2(rin, q3[f], z3);
         }
     }
-    d2 pre[M0];
+    d2 pre[M0], pre2[M0];  // q planes t+1 and t+2 in flight
     if (tq0 + 1 < tq1) {
         const int zo = wrapz(tq0 + 1, nk) * iskb;
 #pragma unroll
         for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+    }
+    if (tq0 + 2 < tq1) {
+        const int zo = wrapz(tq0 + 2, nk) * iskb;
+#pragma unroll
+        for (int m = 0; m < M0; ++m) pre2[m] = ld2(rin, off0[m], zo);
     }
     __syncthreads();
 
@@ -975,11 +988,14 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
         __builtin_amdgcn_sched_barrier(0);
         if (t + 1 < tq1) {
 #pragma unroll
-            for (int m = 0; m < M0; ++m) *(d2*)&P0[prv][dst0[m]] = pre[m];
-            if (t + 2 < tq1) {
-                const int zo = wrapz(t + 2, nk) * iskb;
+            for (int m = 0; m < M0; ++m)
+                if (wr0(m)) *(d2*)&P0[prv][dst0[m]] = pre[m];
 #pragma unroll
-                for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+            for (int m = 0; m < M0; ++m) pre[m] = pre2[m];
+            if (t + 3 < tq1) {
+                const int zo = wrapz(t + 3, nk) * iskb;
+#pragma unroll
+                for (int m = 0; m < M0; ++m) pre2[m] = ld2(rin, off0[m], zo);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -988,8 +1004,9 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
             quad_step<1, R & 3>(P0[cur], w, acc1, d1, c1);
 #pragma unroll
             for (int f = 0; f < 2; ++f) {
-                *(d2*)&P1[cur][o1 + f * W1] =
-                    d2{fmaT(w.f1, d1[2 * f], r1b[f].x), fmaT(w.f1, d1[2 * f + 1], r1b[f].y)};
+                if (live1)
+                    *(d2*)&P1[cur][o1 + f * W1] =
+                        d2{fmaT(w.f1, d1[2 * f], r1b[f].x), fmaT(w.f1, d1[2 * f + 1], r1b[f].y)};
                 r1b[f] = r1a[f];
                 r1a[f] = c1[f];
             }
