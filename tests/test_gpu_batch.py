@@ -116,3 +116,33 @@ def test_batch_uneven_sizes_converge_to_serial_fine(sp):
         assert e <= 1e-8, (i, e)
         assert all(y < x or y <= 1e-13 for x, y in zip(inc, inc[1:]))
     assert rep.iterations <= K
+
+
+def test_batch_mixed_kernels_match_oracle(sp):
+    # sizes that put different kernels in one batch: 2x2 blocks (96^3 x 32 slices: 288
+    # blocks of 32x32) and pair 32x16 (fewer blocks); n is even in a batch
+    K = 2
+    spec = [(96, 32, 0.01, (0.9, -0.4, 0.3), 0.002, 64, 32),
+            (48, 12, 0.02, (-0.5, 0.7, 0.2), 0.003, 24, 6),
+            (34, 4, 0.015, (0.2, 0.3, -0.8), 0.004, 16, 4),
+            (40, 20, 0.01, (0.6, 0.1, 0.5), 0.002, 40, 20)]
+    probs = [sp.Problem(n, S, nu, c, T, sf, sc) for n, S, nu, c, T, sf, sc in spec]
+    b = sp.Batch(probs, kmax=K, coarse_dtype="f64", use_graphs=False)
+    u0, q = b.fields("u"), b.fields("q")
+    for i, (f, p) in enumerate(zip(u0, probs)):
+        f.set(gauss(p.n, i))
+    b.run(u0, q)
+    variants = set()
+    for i, p in enumerate(probs):
+        variants.add(b.info(i).fine_variant)
+        conf = RuntimeConfiguration(gridSize=p.n, endTime=p.end_time, timeStepsFine=p.steps_fine,
+                                    timeStepsCoarse=p.steps_coarse, nu0=p.nu, nufreq=0.0, cx=p.c[0], cy=p.c[1],
+                                    cz=p.c[2], kmax=K, async_=False, timeSlices=p.slices)
+        oc = O.OConvection(p.n, p.n, p.n, 1.0 / p.n, p.nu, 0.0, *p.c)
+        out, _ = oc.parareal(gauss(p.n, i), p.slices, K, p.end_time, conf.dtFine(),
+                             conf.timeStepsFinePerTimeSlice(), conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(),
+                             "rk4", keep_iters=False)
+        e = relerr(q[i].get(), out)
+        assert e <= 1e-12, (i, e)
+    assert variants == {1, 2}, variants  # grouped pair and 2x2 kernels both ran
+

@@ -123,6 +123,22 @@ def test_fused_rk4_kchunks_and_nonsquare(sp, kchunk, variant):
     assert relerr(b.get(), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("variant", ["pair", "quad"])
+@pytest.mark.parametrize("shape", [(40, 27, 20), (34, 9, 11), (64, 33, 40)])
+def test_fused_rk4_odd_y_extent_and_reference_layout(sp, shape, variant):
+    # odd nj: block rows of the 2x2 kernel straddle the periodic wrap in y; partial tiles
+    # in x and y; input in the compact layout, output in the reference CUDA layout
+    ni, nj, nk = shape
+    q0 = rand_field((nk, nj, ni), ni + nj)
+    conv = sp.Convection(ni, nj, nk, 1. / 32, 0.02, 0.0, *C3)
+    conv.set_tile(32, 32, 512, 0, variant=variant)
+    a, b = conv.field(), conv.field(kind="stella")
+    a.set(q0)
+    conv.DoRK4(a, b, 0.0, 1e-4, 3)
+    ref = O.OConvection(ni, nj, nk, 1. / 32, 0.02, 0.0, *C3).rk4(q0, 0.0, 1e-4, 3)
+    assert relerr(b.get(), ref) <= 1e-12
+
+
 @pytest.mark.parametrize("steps", [0, 1, 2, 3])
 def test_fused_rk4_aliasing_and_zero_steps(sp, steps):
     N = 16
