@@ -114,13 +114,15 @@ int sp_prop_set_path(sp_prop* p, int path);
 /* Fused-kernel choice: variant SP_KERNEL_*, tile tx x ty columns, nthreads per block,
  * kchunk planes per block (0: whole k extent).  variant SP_KERNEL_AUTO restores the
  * built-in choice.  Compiled shapes: scalar {32x32x1024, 32x24x1024, 24x24x1024,
- * 32x16x512, 16x16x256}, pair {32x32x512, 32x16x512, 16x16x256}. */
+ * 32x16x512, 16x16x256}, pair {32x32x512, 32x16x512, 16x16x256}, quad {32x32x512}
+ * (the built-in choice for FP64 launches of 256 or more blocks). */
 int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int kchunk);
 /* The fused-kernel tile sp_do_rk4 (nslices = 1) or sp_do_rk4_batched (nslices > 1) will
  * use for FP64 compute (f64 = 1) or reduced-precision storage (f64 = 0): the one set with
  * sp_prop_set_tile, else the built-in choice for this size.  Output pointers may be NULL.
- * Fields whose sizes or alignment the pair kernel cannot take run the scalar 32x16x512
- * kernel instead (no counterpart in the reference; reports what is launched). */
+ * Fields the pair and quad kernels cannot take (odd ni, odd strides, origins not 16-byte
+ * aligned) run the scalar 32x16x512 kernel instead; the report covers odd ni (no
+ * counterpart in the reference). */
 int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* tx, int* ty, int* nthreads,
                      int* kchunk);
 /* Autotuner (no counterpart in the reference, whose CUDA block size is fixed at 32x4):

@@ -179,6 +179,7 @@ def test_odd_sizes_fall_back_to_scalar(sp, variant):
     conv.DoRK4(a, b, 0.0, 1e-4, 2)
     ref = O.OConvection(ni, nj, nk, 1. / 21, 0.01, 0.0, *C3).rk4(q0, 0.0, 1e-4, 2)
     assert relerr(b.get(), ref) <= 1e-12
+    assert conv.get_tile(1, True) == {"variant": "scalar", "tx": 32, "ty": 16, "nthreads": 512, "kchunk": 0}
 
 
 @pytest.mark.parametrize("nk", [2, 4, 7])
