@@ -1034,10 +1034,16 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
     // U_r = G^{r*tsG}(u0), one continuous coarse run from t = 0 (Parareal.h:113): rank r's own
     // run visits the same states at the same times (step j at t = 0 + j*dtG), so its value is
     // the snapshot after r*tsG steps.  The chain state stays in coarse precision and
-    // alternates between Gw0 and Gw1; slice 0 of Gold (filled below) is the sub-step scratch.
+    // alternates between Gw0 and Gw1; slice 0 of FS is the sub-step scratch.
+    // Gold_r = G(U_r) (Parareal.h:114) is the chain's next snapshot whenever nu is constant:
+    // the same coarse steps on the same values (the kernel variants agree bitwise), only the
+    // times differ, and they enter only through nu.  The chain then writes Gold itself and
+    // runs one more segment for Gold_{S-1}; otherwise Gold is one batched run below.
+    const bool gold_from_chain = !pr->cfg.delta_correction && pr->nu_const;
     {
         const View bufs[2] = {cview(pr->Gw0.p, p->ni, p->nj), cview(pr->Gw1.p, p->ni, p->nj)};
-        const View Z = cview(pr->Gold.p, p->ni, p->nj);
+        const View Z = cview(pr->FS.p, p->ni, p->nj);
+        const auto gold = [&](int r) { return cview((char*)pr->Gold.p + (size_t)r * pr->n * gs, p->ni, p->nj); };
         int cur = -1;
         for (int r = 1; r < pr->S; ++r) {
             const int nxt = cur < 0 ? 0 : 1 - cur;
@@ -1047,11 +1053,17 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
             cur = nxt;
             SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt, bufs[cur], SP_F64,
                                   cview((char*)pr->U.p + (size_t)r * pr->n * 8, p->ni, p->nj)));
+            if (gold_from_chain) SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, pr->gdt, bufs[cur], pr->gdt, gold(r - 1)));
+        }
+        if (gold_from_chain) {
+            const View src = cur < 0 ? cview(pr->U.p, p->ni, p->nj) : bufs[cur];
+            SP_TRY(coarse_run(pr, 1, pr->S - 1, cur < 0 ? SP_F64 : pr->gdt, src, gold(pr->S - 1), Z, nullptr,
+                              (long long)(pr->S - 1) * pr->tsG, pr->tsG));
         }
     }
-    // Gold_r = G(U_r) from tStart_r (Parareal.h:114), all slices in one batch; FS is scratch.
+    // Otherwise Gold_r = G(U_r) from tStart_r, all slices in one batch; FS is scratch.
     // (The difference form needs no coarse values.)
-    if (!pr->cfg.delta_correction) {
+    if (!pr->cfg.delta_correction && !gold_from_chain) {
         View src = batch_view(pr->U.p, pr, SP_F64);
         View dst = batch_view(pr->Gold.p, pr, pr->gdt);
         View scr = batch_view(pr->FS.p, pr, pr->gdt);
