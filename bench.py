@@ -278,8 +278,11 @@ def main():
     t_steps = time.perf_counter() - t0
     ms = t_steps / a.steps * 1e3
 
+    # updates the solve performs: K fine sweeps over S slices; coarse: the initial chain (S
+    # segments: S-1 slice starts, and the last slice's coarse value, which with constant nu
+    # also supplies every other G(U_r^0)) and S-1 segments per iteration (G(U_0) is reused)
     lup_f = w.kmax * S * tsF * N ** 3
-    lup_g = tsG * N ** 3 * ((S - 1) + S + w.kmax * (S - 1))
+    lup_g = tsG * N ** 3 * (S + w.kmax * (S - 1))
     value = (lup_f + lup_g) / (ms / 1e3) / 1e9
 
     # error of every iterate against the serial fine solution (untimed)
