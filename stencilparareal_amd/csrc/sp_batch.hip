@@ -336,6 +336,23 @@ int sp_batch_create(sp_ctx* ctx, int nprob, const sp_problem* probs, const sp_ba
     };
     size_t part_total = 0;
     std::map<std::vector<int>, FusedTile> tuned;  // (n, slices, f64) -> tile, one tuning per shape
+    // The fine sweep is one grouped launch per tile shape in use, and a problem's own choice
+    // is made for the problem alone (k chunks and narrow tiles fill the device for a small
+    // problem; inside a grouped launch they only add pipeline fill and split the step into
+    // several launches).  Once whole-k 2x2 tiles over all problems give several blocks per
+    // CU, every problem's fine step uses them.
+    bool shared_fine = false;
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus < 1)
+            cus = 256;
+        long long blocks = 0;
+        for (int p = 0; p < nprob; ++p) {
+            const long long t = (probs[p].n + 31) / 32;
+            blocks += t * t * probs[p].slices;
+        }
+        shared_fine = blocks >= 4LL * cus;
+    }
     for (int p = 0; p < nprob && r == SP_OK; ++p) {
         Prob& q = b->pr[p];
         q.d = probs[p];
@@ -377,7 +394,10 @@ int sp_batch_create(sp_ctx* ctx, int nprob, const sp_problem* probs, const sp_ba
             t.variant = v;
             tuned[key] = t;
         };
-        pick(q.S, SP_F64, q.tileF);
+        if (shared_fine)
+            q.tileF = FusedTile{32, 32, 512, 0, SP_KERNEL_QUAD};
+        else
+            pick(q.S, SP_F64, q.tileF);
         pick(1, b->gdt, q.tileG);
     }
     if (r == SP_OK) r = b->part.ensure(part_total * sizeof(double));
