@@ -805,7 +805,10 @@ struct QuadMap {
     }
 };
 
-// One stage at one plane for a 2x2 block; point p = 2 f + e (row f, column e).
+// One stage at one plane for a 2x2 block; point p = 2 f + e (row f, column e).  Each group
+// of LDS reads is issued at raised wave priority: of the two waves sharing a SIMD, the one
+// about to wait on LDS gets its reads out first while the other computes (-3 to -4 % per
+// step at 16 x 128^3, 8 x 160^3, 24 x 96^3).
 template <int S, int R>
 __device__ __forceinline__ void quad_step(const double* __restrict__ P, const StarW<double>& w, double (&acc)[4][4],
                                           double (&done)[4], d2 (&cen)[2])
@@ -817,7 +820,9 @@ __device__ __forceinline__ void quad_step(const double* __restrict__ P, const St
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
         // centres and x neighbours of row f, and the z sums
+        __builtin_amdgcn_s_setprio(1);
         const d2 C = *(const d2*)(c + f * W), Lx = *(const d2*)(c + f * W - 2), Rx = *(const d2*)(c + f * W + 2);
+        __builtin_amdgcn_s_setprio(0);
         cen[f] = C;
         const double u[2] = {C.x, C.y};
         const double xm2[2] = {Lx.x, Lx.y}, xm1[2] = {Lx.y, C.x}, xp1[2] = {C.y, Rx.x}, xp2[2] = {Rx.x, Rx.y};
@@ -838,8 +843,10 @@ __device__ __forceinline__ void quad_step(const double* __restrict__ P, const St
     }
     {
         // y neighbours: rows y-2, y-1, y+2, y+3 and the two centre rows
+        __builtin_amdgcn_s_setprio(1);
         const d2 Ym2 = *(const d2*)(c - 2 * W), Ym1 = *(const d2*)(c - W);
         const d2 Yp2 = *(const d2*)(c + 2 * W), Yp3 = *(const d2*)(c + 3 * W);
+        __builtin_amdgcn_s_setprio(0);
         const double r[6][2] = {{Ym2.x, Ym2.y}, {Ym1.x, Ym1.y}, {cen[0].x, cen[0].y},
                                 {cen[1].x, cen[1].y}, {Yp2.x, Yp2.y}, {Yp3.x, Yp3.y}};
 #pragma unroll
