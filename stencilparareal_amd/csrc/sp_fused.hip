@@ -467,6 +467,8 @@ struct PairMap {
         at(g, ax, by);
         return by * G::WO + ax;
     }
+    // slot g is a pair of its own: not past the last slot, not a row's padding slot
+    __device__ static bool live(int g) { return g < SLOTS && g % VW < WP; }
 };
 
 template <int TX, int TY, int S, int NT, int M, int R, int WI = Geo<TX, TY, S>::WI>
@@ -615,6 +617,22 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         o4[m] = (pmod(x0 + ax, ni) + pmod(y0 + by, nj) * osj) * 8;  // periodic image for partial tiles
     }
     int s2[M2], w3[M2], r3[M3], w4[M3], r4[M4], n4[M4];
+    // slots past a stage's last pair and the padding slots of padded rows repeat a pair:
+    // they write nothing (several lanes writing one LDS address serialise).  The 32x32
+    // tile has no registers left for the masks and writes the repeats.
+    constexpr bool MASK = TX * TY <= 512;
+    bool lv1[M1], lv2[M2], lv3[M3], lv4[M4];
+    if constexpr (MASK) {
+#pragma unroll
+        for (int m = 0; m < M1; ++m) lv1[m] = PM1::live(tid + m * NT);
+#pragma unroll
+        for (int m = 0; m < M2; ++m) lv2[m] = PM2::live(tid + m * NT);
+#pragma unroll
+        for (int m = 0; m < M3; ++m) lv3[m] = PM3::live(tid + m * NT);
+#pragma unroll
+        for (int m = 0; m < M4; ++m) lv4[m] = PM4::live(tid + m * NT);
+    }
+    const auto wr0 = [&](int m) { return !MASK || m < M0 - 1 || tid + m * NT < A0P; };  // loader tail
     pair_qidx<TX, TY, 2, NT, M2>(s2, w3);
     pair_qidx<TX, TY, 3, NT, M3>(r3, w4);
     pair_qidx<TX, TY, 4, NT, M4>(r4, n4);
@@ -653,7 +671,7 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         const int zo = wrapz(tq0, nk) * iskb;
 #pragma unroll
         for (int m = 0; m < M0; ++m)
-            *(d2*)&P0[tq0 & 1][2 * min(tid + m * NT, A0P - 1)] = ld2(rin, off0[m], zo);
+            if (wr0(m)) *(d2*)&P0[tq0 & 1][2 * min(tid + m * NT, A0P - 1)] = ld2(rin, off0[m], zo);
         pair_qload<M2>(qv2, rin, wrapz(z0 - 8, nk) * iskb, q2);
     }
     d2 pre[M0];
@@ -670,7 +688,8 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         __builtin_amdgcn_sched_barrier(0);  // keep planes apart: bounded live ranges
         if (t + 1 < tq1) {
 #pragma unroll
-            for (int m = 0; m < M0; ++m) *(d2*)&P0[prv][2 * min(tid + m * NT, A0P - 1)] = pre[m];
+            for (int m = 0; m < M0; ++m)
+                if (wr0(m)) *(d2*)&P0[prv][2 * min(tid + m * NT, A0P - 1)] = pre[m];
             if (t + 2 < tq1) {
                 const int zo = wrapz(t + 2, nk) * iskb;
 #pragma unroll
@@ -683,8 +702,9 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
             pair_step<TX, TY, 1, NT, M1, R & 3>(P0[cur], w, acc1, d1, c1);
 #pragma unroll
             for (int m = 0; m < M1; ++m) {
-                *(d2*)&P1[cur][PM1::out(tid + m * NT)] =
-                    d2{fmaT(w.f1, d1[m][0], r1b[m].x), fmaT(w.f1, d1[m][1], r1b[m].y)};
+                if (!MASK || lv1[m])
+                    *(d2*)&P1[cur][PM1::out(tid + m * NT)] =
+                        d2{fmaT(w.f1, d1[m][0], r1b[m].x), fmaT(w.f1, d1[m][1], r1b[m].y)};
                 r1b[m] = r1a[m];
                 r1a[m] = c1[m];
             }
@@ -694,9 +714,11 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
             pair_step<TX, TY, 2, NT, M2, (R + 1) & 3>(P1[prv], w, acc2, d2v, c2);
 #pragma unroll
             for (int m = 0; m < M2; ++m) {
-                *(d2*)&P2[cur][PM2::out(tid + m * NT)] =
-                    d2{fmaT(w.f2, d2v[m][0], qv2[m].x), fmaT(w.f2, d2v[m][1], qv2[m].y)};
-                *(d2*)&Q3[cur][w3[m]] = qv2[m];  // q(t-5) for stage 3
+                if (!MASK || lv2[m]) {
+                    *(d2*)&P2[cur][PM2::out(tid + m * NT)] =
+                        d2{fmaT(w.f2, d2v[m][0], qv2[m].x), fmaT(w.f2, d2v[m][1], qv2[m].y)};
+                    *(d2*)&Q3[cur][w3[m]] = qv2[m];  // q(t-5) for stage 3
+                }
             }
             pair_qload<M2>(qv2, rin, wrapz(t - 4, nk) * iskb, q2);
             __builtin_amdgcn_sched_barrier(0);
@@ -706,9 +728,11 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 #pragma unroll
             for (int m = 0; m < M3; ++m) {
                 const d2 qn = *(const d2*)&Q3[prv][r3[m]];  // q(t-6), written by stage 2 last plane
-                *(d2*)&P3[cur][PM3::out(tid + m * NT)] =
-                    d2{fmaT(w.f3, d3[m][0], r3b[m].x), fmaT(w.f3, d3[m][1], r3b[m].y)};
-                *(d2*)&Q4[cur][w4[m]] = r3b[m];  // q(t-8) for stage 4
+                if (!MASK || lv3[m]) {
+                    *(d2*)&P3[cur][PM3::out(tid + m * NT)] =
+                        d2{fmaT(w.f3, d3[m][0], r3b[m].x), fmaT(w.f3, d3[m][1], r3b[m].y)};
+                    *(d2*)&Q4[cur][w4[m]] = r3b[m];  // q(t-8) for stage 4
+                }
                 r3b[m] = r3a[m];
                 r3a[m] = qn;
             }
@@ -721,7 +745,8 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
                 const int zo = zp * oskb;
 #pragma unroll
                 for (int m = 0; m < M4; ++m)
-                    st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], r4b[m].x), fmaT(w.f4, d4[m][1], r4b[m].y)});
+                    if (!MASK || lv4[m])
+                        st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], r4b[m].x), fmaT(w.f4, d4[m][1], r4b[m].y)});
             }
 #pragma unroll
             for (int m = 0; m < M4; ++m) {
