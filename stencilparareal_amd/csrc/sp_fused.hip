@@ -31,15 +31,15 @@
 //  * HBM traffic per step: one read and one write of the state (2*sizeof(storage) bytes
 //    per lattice update) plus the x-y halo overlap of neighbouring tiles.
 //  * The plane for iteration t+1 is loaded into registers while iteration t computes.
+// That is the scalar kernel (any sizes and storage types).  The FP64 x-pair and 2x2
+// variants further down keep the scheme and change the lane layout (16-byte accesses,
+// shared neighbour reads), the way q reaches each stage and the prefetch depth; their
+// own comments give the details.
 
 #include <algorithm>
 #include <cstring>
 
 #include "sp_internal.hpp"
-
-#ifndef SP_SERIAL_SLOTS
-#define SP_SERIAL_SLOTS 1
-#endif
 
 namespace sp {
 
@@ -159,7 +159,7 @@ __device__ __forceinline__ void stage_step(const T* __restrict__ P, const StarW<
         acc[m][sp1] = fmaT(w.zm1, u, acc[m][sp1]);        // point p+1
         // one slot's neighbour values in flight at a time keeps the register footprint
         // small enough for 16 waves per CU; the other waves hide the LDS latency
-        if (SP_SERIAL_SLOTS) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
