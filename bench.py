@@ -12,8 +12,11 @@ workload with its input already resident in device memory.  `value` is the stenc
 throughput of the solve: lattice-point updates (fine RK4 steps + coarse RK4 steps) per
 second.  The line also carries the serial-fine wall time on the same GPU, the Parareal
 speedup over it, the error of each iterate against the serial FP64 fine solution, the
-roofline of the dominant kernel (the batched fused RK4 step), and the CPU baseline (the
-oracle's serial fine sweep on the host cores, a bounded sample).
+wall time to 1e-6 of batched Parareal with FP64 and FP32 coarse propagators against plain
+all-FP64 Parareal in the reference's execution structure (reference-order sweeps, slice
+after slice) on the same GPU, the roofline of the dominant kernel (the batched fused RK4
+step), and the CPU baseline (the oracle's serial fine sweep on the host cores, a bounded
+sample).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -339,6 +342,33 @@ def main():
         extra["parareal_f32_coarse_s"] = round(time.perf_counter() - t0, 4)
         extra["parareal_f32_coarse_rel_l2_at_K"] = sp.rel_l2(sol2, serial)
         del pr32
+
+        # The configuration's own target: batched Parareal with FP32 G against plain all-FP64
+        # Parareal on the same GPU, both run until the 1e-6 increment test stops them.  The
+        # plain one is the reference's execution structure: reference-order sweeps (4 RHS + 3
+        # Euler + 1 RK kernel per step), slice after slice, FP64 G.
+        def to_tol_run(c, gdt, field_name, graphs):
+            sol = c.field(field_name)
+            pr_x = sp.Parareal(c, init, sol, conf, coarse="rk4", coarse_dtype=gdt, use_graphs=graphs, tol=1e-6)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            rep_x = pr_x.DoParallel()
+            torch.cuda.synchronize()
+            out_x = {"wall_s": round(time.perf_counter() - t1, 4), "iterations": rep_x.iterations,
+                     "rel_l2_err": float(f"{sp.rel_l2(sol, serial):.3e}")}
+            del pr_x
+            return out_x
+        log("batched FP32-G Parareal to tol 1e-6")
+        to_tol_run(conv, "f32", "q_f32t", bool(a.graphs))  # warm-up (graph capture, first launches)
+        mixed = to_tol_run(conv, "f32", "q_f32t", bool(a.graphs))
+        log("plain all-FP64 Parareal to tol 1e-6 (reference-order sweeps, slice by slice)")
+        conv_plain = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz, path="staged")
+        plain = to_tol_run(conv_plain, "f64", "q_plain", False)  # host-launched, as the reference
+        del conv_plain
+        extra["target_to_tol_1e-6"] = {
+            "batched_f32_coarse": mixed, "plain_fp64_reference_structure": plain,
+            "speedup": round(plain["wall_s"] / mixed["wall_s"], 3),
+            "speedup_batched_f64_coarse": round(plain["wall_s"] / t_tol, 3)}
 
     cpu = None
     if not a.no_cpu:
