@@ -337,6 +337,22 @@ def test_parareal_staged_bitwise_every_iterate(sp, coarse, nufreq):
 
 
 @pytest.mark.parametrize("coarse", ["euler", "rk4"])
+@pytest.mark.parametrize("S", [1, 2])
+def test_parareal_few_slices_bitwise(sp, coarse, S):
+    # one and two slices: the initial coarse run supplies every slice's coarse value (a
+    # single slice takes it from one segment started at u0)
+    conf, conv, u0, iters, _ = _parareal_case(sp, "staged", coarse, S=S, kmax=2)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse=coarse)
+    pr.begin()
+    assert np.array_equal(pr.solution().get(), iters[0])
+    for k in range(1, conf.kmax + 1):
+        pr.iterate()
+        assert np.array_equal(pr.solution().get(), iters[k]), k
+
+
+@pytest.mark.parametrize("coarse", ["euler", "rk4"])
 @pytest.mark.parametrize("graphs", [False, True])
 def test_parareal_fused_vs_oracle(sp, coarse, graphs):
     conf, conv, u0, iters, serial = _parareal_case(sp, "fused", coarse)
