@@ -54,9 +54,27 @@ def cpu_baseline(w, budget_s=12.0):
     t0 = O.wtime()
     conv.rk4(q0, 0.0, dt, n)
     t = O.wtime() - t0
-    return {"value": round(N ** 3 * n / t / 1e9, 5), "unit": "GLUP/s", "cores": O.max_threads(),
+    threads = O.max_threads()
+    # the same sweep on one thread, a shorter sample
+    O.set_threads(1)
+    try:
+        n1 = max(2, int(3.0 / (t / n * threads)))
+        t1 = O.wtime()
+        conv.rk4(q0, 0.0, dt, n1)
+        t1 = O.wtime() - t1
+    finally:
+        O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": round(N ** 3 * n / t / 1e9, 5), "unit": "GLUP/s", "cores": threads,
             "kind": "port",
-            "sample": f"serial fine RK4 (oracle, reference OpenMP structure), {N}^3, {n} steps, {t:.2f} s"}
+            "sample": f"serial fine RK4 (oracle, reference OpenMP structure), {N}^3, {n} steps, {t:.2f} s",
+            "one_thread": {"value": round(N ** 3 * n1 / t1 / 1e9, 5), "sample": f"{n1} steps, {t1:.2f} s"},
+            "nproc": os.cpu_count(), "cpu_model": model}
 
 
 def measure_batched_step(conv, init, conf, S, reps=50, copy_check=False):

@@ -72,6 +72,7 @@ def lib():
             "orc_fill_gaussians": (None, [_dp] + [C.c_int] * 4 + [_dp] * 5),
             "orc_wtime": (C.c_double, []),
             "orc_max_threads": (C.c_int, []),
+            "orc_set_threads": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -233,3 +234,8 @@ def wtime() -> float:
 
 def max_threads() -> int:
     return lib().orc_max_threads()
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads for later oracle calls (the CPU baseline's one-thread sample)."""
+    lib().orc_set_threads(int(n))

@@ -84,6 +84,7 @@ void orc_fill_gaussians(double* compact, int ni, int nj, int nk, int ng,
 
 double orc_wtime(void);
 int orc_max_threads(void);
+void orc_set_threads(int n); /* OpenMP threads of later calls (n > 0) */
 
 #ifdef __cplusplus
 }
