@@ -2,10 +2,16 @@
 //
 // Mirrors the part of the reference's ConvectionField (src/Convection/Convection.h:13-20;
 // the STELLA DataFieldCUDA API) that src/Parareal/main.cpp and Parareal.h use:
-//   Init(name, domain, kboundary), calculationDomain(), operator()(i, j, k) on the host
-//   copy, SynchronizeHostStorage(), SynchronizeDeviceStorage(), SwapWith(), copy.
+//   Init(name, domain, kboundary), calculationDomain(), boundary(), storage().pStorageBase(),
+//   storage().paddedSize(), operator()(i, j, k) on the host copy, SynchronizeHostStorage(),
+//   SynchronizeDeviceStorage(), SwapWith(), copy.
 // Storage is the reference GPU layout (KJI order, i padded to 32 doubles, 2-cell halo), on
-// the device through the C ABI; a host mirror holds the same bytes for element access.
+// the device through the C ABI; pStorageBase() is the device address, as on the reference's
+// CUDA backend (DataFieldCUDAStorage.h:123).  A host mirror holds the same bytes for element
+// access and follows the device lazily: element writes mark the mirror newer, and the next
+// device use uploads it; handing out the device address marks the device newer, and the
+// next element access downloads it.  So call sites that skip the explicit Synchronize*
+// calls (the reference's CPU-backend form of main.cpp) still see consistent data.
 #pragma once
 
 #include <cstdlib>
@@ -37,6 +43,9 @@ inline sp_ctx* context()
     return ctx;
 }
 
+// Halo width of every ConvectionField (src/Convection/Convection.h:13).
+static const int convectionBoundaryLines = 2;
+
 struct IJKSize {
     int i = 0, j = 0, k = 0;
     void Init(int isize, int jsize, int ksize)
@@ -51,16 +60,57 @@ struct IJKSize {
 };
 
 struct KBoundary {
-    int m = -2, p = 2;
+    int m = -convectionBoundaryLines, p = convectionBoundaryLines;
     void Init(int kminus, int kplus)
     {
         m = kminus;
         p = kplus;
     }
+    int kMinusOffset() const { return m; }
+    int kPlusOffset() const { return p; }
+};
+
+// Halo offsets of a field on each side (negative on the minus sides), as STELLA reports them.
+struct IJKBoundary {
+    int im = 0, ip = 0, jm = 0, jp = 0, km = 0, kp = 0;
+    void Init(int iMinus, int iPlus, int jMinus, int jPlus, int kMinus, int kPlus)
+    {
+        im = iMinus;
+        ip = iPlus;
+        jm = jMinus;
+        jp = jPlus;
+        km = kMinus;
+        kp = kPlus;
+    }
+    int iMinusOffset() const { return im; }
+    int iPlusOffset() const { return ip; }
+    int jMinusOffset() const { return jm; }
+    int jPlusOffset() const { return jp; }
+    int kMinusOffset() const { return km; }
+    int kPlusOffset() const { return kp; }
 };
 
 class ConvectionField {
 public:
+    // What field.storage() returns: the device storage's base address and padded extent.
+    class Storage {
+    public:
+        explicit Storage(ConvectionField& f) : f_(f) {}
+        // first element of the allocation (the halo corner), device memory
+        double* pStorageBase() const { return static_cast<double*>(f_.device()); }
+        // allocated extent: i padded to 32 doubles incl. halo, j and k incl. halo
+        IJKSize paddedSize() const
+        {
+            IJKSize s;
+            const sp_layout& l = f_.layout_;
+            s.Init((int)l.sj, (int)(l.sk / l.sj), l.nk + 2 * l.halo);
+            return s;
+        }
+
+    private:
+        ConvectionField& f_;
+    };
+
     ConvectionField() = default;
     ConvectionField(const ConvectionField& o) { *this = o; }
     ConvectionField& operator=(const ConvectionField& o)
@@ -69,12 +119,12 @@ public:
         release();
         if (o.dev_) {
             Init(o.name_, o.domain_, KBoundary{});
-            check(sp_ctx_sync(context()));
-            // device copy of the whole storage (the reference's copy constructor copies data)
-            std::vector<double> tmp(elems_);
-            check(sp_memcpy_d2h(context(), tmp.data(), o.dev_, (long long)(elems_ * sizeof(double))));
-            check(sp_memcpy_h2d(context(), dev_, tmp.data(), (long long)(elems_ * sizeof(double))));
-            host_ = tmp;
+            // a full copy of the storage (the reference's copy constructor copies data)
+            o.pull();
+            host_ = o.host_;
+            dev_newer_ = false;
+            host_newer_ = true;
+            push();
         }
         return *this;
     }
@@ -91,27 +141,61 @@ public:
         check(sp_malloc(context(), (long long)(elems_ * sizeof(double)), &p));
         dev_ = p;
         host_.assign(elems_, 0.);
-        check(sp_memcpy_h2d(context(), dev_, host_.data(), (long long)(elems_ * sizeof(double))));
+        host_newer_ = true;
+        dev_newer_ = false;
+        push();
     }
 
     const std::string& name() const { return name_; }
     const IJKSize& calculationDomain() const { return domain_; }
+    IJKBoundary boundary() const
+    {
+        const int h = layout_.halo;
+        IJKBoundary b;
+        b.Init(-h, h, -h, h, -h, h);
+        return b;
+    }
+    Storage storage() { return Storage(*this); }
     const sp_layout& layout() const { return layout_; }
-    void* device() { return dev_; }
-    const void* device() const { return dev_; }
+    // Device address for a kernel call: uploads pending element writes; the device copy is
+    // then taken as the newer one.
+    void* device()
+    {
+        push();
+        dev_newer_ = true;
+        return dev_;
+    }
+    const void* device() const
+    {
+        push();
+        return dev_;
+    }
 
     // host element access (interior coordinates, halo cells at -2..-1 and n..n+1)
-    double& operator()(int i, int j, int k) { return host_[index(i, j, k)]; }
-    double operator()(int i, int j, int k) const { return host_[index(i, j, k)]; }
+    double& operator()(int i, int j, int k)
+    {
+        pull();
+        host_newer_ = true;
+        return host_[index(i, j, k)];
+    }
+    double operator()(int i, int j, int k) const
+    {
+        pull();
+        return host_[index(i, j, k)];
+    }
+
+    // the device copy holds every element write (STELLA's DataFieldCUDA::isDeviceUpToDate)
+    bool isDeviceUpToDate() const { return !host_newer_; }
 
     void SynchronizeHostStorage() const
     {
-        check(sp_ctx_sync(context()));
-        check(sp_memcpy_d2h(context(), const_cast<double*>(host_.data()), dev_, (long long)(elems_ * sizeof(double))));
+        dev_newer_ = true;
+        pull();
     }
     void SynchronizeDeviceStorage()
     {
-        check(sp_memcpy_h2d(context(), dev_, host_.data(), (long long)(elems_ * sizeof(double))));
+        host_newer_ = true;
+        push();
     }
 
     void SwapWith(ConvectionField& o)
@@ -122,12 +206,30 @@ public:
         std::swap(dev_, o.dev_);
         std::swap(elems_, o.elems_);
         std::swap(host_, o.host_);
+        std::swap(host_newer_, o.host_newer_);
+        std::swap(dev_newer_, o.dev_newer_);
     }
 
 private:
     size_t index(int i, int j, int k) const
     {
         return (size_t)(layout_.origin + i + (long long)j * layout_.sj + (long long)k * layout_.sk);
+    }
+    // device -> host mirror, after the work queued on the library stream
+    void pull() const
+    {
+        if (!dev_newer_ || !dev_) return;
+        check(sp_ctx_sync(context()));
+        check(sp_memcpy_d2h(context(), host_.data(), dev_, (long long)(elems_ * sizeof(double))));
+        dev_newer_ = false;
+    }
+    // host mirror -> device (sp_memcpy_h2d is synchronous)
+    void push() const
+    {
+        if (!host_newer_ || !dev_) return;
+        check(sp_ctx_sync(context()));
+        check(sp_memcpy_h2d(context(), dev_, host_.data(), (long long)(elems_ * sizeof(double))));
+        host_newer_ = false;
     }
     void release()
     {
@@ -140,7 +242,9 @@ private:
     sp_layout layout_{};
     void* dev_ = nullptr;
     size_t elems_ = 0;
-    std::vector<double> host_;
+    mutable std::vector<double> host_;
+    mutable bool host_newer_ = false;  // element writes not yet on the device
+    mutable bool dev_newer_ = false;   // the device copy may hold newer data than the mirror
 };
 
 }  // namespace sp

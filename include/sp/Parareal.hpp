@@ -1,15 +1,20 @@
 // Parareal.hpp -- drop-in C++ Parareal driver over the C ABI.
 //
 // Same shape as the reference template (src/Parareal/Parareal.h:38-289):
-//   Parareal<PropagatorT, FieldType>(propagator, initial, solution, conf)
+//   Parareal<PropagatorT, FieldType>(propagator, initial, solution, tStart, conf, comm)
 //   DoParallel(), DoTimedParallel(std::vector<double>&), DoSerial()
-// The reference runs one slice per MPI rank and takes (tStart, comm); here all slices are
-// the batch index of one GPU, so neither is needed.  The solution field receives the last
-// slice's result (the last rank's q).  Extensions select the coarse propagator
-// (DoEuler as in the reference, or RK4 in f64 / f32 / bf16 storage), a stopping tolerance
-// and hipGraph capture of each iteration.
+// The reference runs one slice per MPI rank and takes the rank's start time and the
+// communicator (Parareal.h:48-51, called at src/Parareal/main.cpp:193); here all slices are
+// the batch index of one GPU, so the one process owns the whole horizon: tStart must be 0
+// (rank 0's value, timeSliceSize() * 0) and the communicator is accepted and ignored.  A
+// four-argument form (conf, options) needs neither.  The solution field receives the last
+// slice's result (the last rank's q).  Extensions select the coarse propagator (DoEuler as
+// in the reference, or RK4 in f64 / f32 / bf16 storage), a stopping tolerance and hipGraph
+// capture of each iteration.
 #pragma once
 
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "Convection.hpp"
@@ -28,9 +33,29 @@ struct PararealOptions {
                           // reference's; 0: the reference's; 1: difference form
 };
 
+// The communicator argument of the reference constructor: any value converts to it (an
+// MPI_Comm handle, MPI_COMM_WORLD, or sp::COMM_WORLD where no MPI header is included).
+struct Comm {
+    Comm() = default;
+    template <typename T>
+    Comm(const T&)
+    {
+    }
+};
+static const Comm COMM_WORLD{};
+
 template <typename PropagatorT, typename FieldType>
 class Parareal {
 public:
+    // Parareal.h:48-51.  tStart is the calling rank's slice start; with every slice on this
+    // GPU the caller is rank 0 of a one-rank job, so only 0 is valid (checked before any
+    // device work).
+    Parareal(PropagatorT& propagator, FieldType& initial, FieldType& solution, double tStart,
+             const RuntimeConfiguration& conf, Comm /*comm*/, const PararealOptions& opt = PararealOptions())
+        : Parareal(propagator, initial, solution, start_checked(tStart, conf), opt)
+    {
+    }
+
     Parareal(PropagatorT& propagator, FieldType& initial, FieldType& solution, const RuntimeConfiguration& conf,
              const PararealOptions& opt = PararealOptions())
         : initial_(initial), q_(solution), kmax_(conf.kmax())
@@ -90,6 +115,15 @@ public:
     const sp_parareal_report& report() const { return report_; }
 
 private:
+    static const RuntimeConfiguration& start_checked(double tStart, const RuntimeConfiguration& conf)
+    {
+        if (tStart != 0.)
+            throw std::invalid_argument("stencilparareal: Parareal tStart must be 0 (all " +
+                                        std::to_string(conf.timeSlices()) +
+                                        " time slices run on this device), got " + std::to_string(tStart));
+        return conf;
+    }
+
     FieldType& initial_;
     FieldType& q_;
     int kmax_;

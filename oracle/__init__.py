@@ -63,7 +63,7 @@ def lib():
             "orc_rhs2": (None, [_vp, _vp, _vp, C.c_double]),
             "orc_computenu": (C.c_double, [_vp, C.c_double]),
             "orc_parareal_parallel": (None, [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
-                                             C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]),
+                                             C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int]),
             "orc_parareal_serial": (None, [_vp, C.c_double, C.c_int, _dp, _dp]),
             "orc_error_reference": (C.c_double, [_dp, _dp, C.c_long]),
             "orc_rel_l2": (C.c_double, [_dp, _dp, C.c_long]),
@@ -167,15 +167,17 @@ class OConvection:
     def computenu(self, t: float) -> float:
         return lib().orc_computenu(self.h, t)
 
-    def parareal(self, u0, S, kmax, T, dtF, tsF, dtG, tsG, g_kind="euler", keep_iters=True):
-        """Emulated ``Parareal::DoParallel`` (blocking schedule).  Returns
-        (final, iters) with iters[k] the last rank's q after k iterations."""
+    def parareal(self, u0, S, kmax, T, dtF, tsF, dtG, tsG, g_kind="euler", keep_iters=True,
+                 async_=False):
+        """Emulated ``Parareal::DoParallel`` (blocking schedule, or the async schedule's
+        fine-step times with async_=True).  Returns (final, iters) with iters[k] the last
+        rank's q after k iterations."""
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         out = np.empty(self.shape, dtype=np.float64)
         iters = np.empty((kmax + 1,) + self.shape, dtype=np.float64) if keep_iters else None
         g = {"euler": 0, "rk4": 1}[g_kind]
         lib().orc_parareal_parallel(self.h, S, kmax, T, dtF, tsF, dtG, tsG, g, _p(u0),
-                                    _p(iters) if iters is not None else None, _p(out))
+                                    _p(iters) if iters is not None else None, _p(out), 1 if async_ else 0)
         return out, iters
 
     def serial(self, u0, dtF, steps_total):

@@ -371,12 +371,14 @@ static void update_sweep(const orc_conv* c, orc_field* q, const orc_field* fine,
     SWEEP_END
 }
 
-/* Parareal::DoParallel (Parareal.h:110-171) with the blocking schedule
- * (--noasync), emulated on S ranks in rank order.  Rank r starts at
- * tStart_r = (T/S)*r (src/Parareal/main.cpp:157). */
+/* Parareal::DoParallel (Parareal.h:110-171), emulated on S ranks in rank order.  Rank r
+ * starts at tStart_r = (T/S)*r (src/Parareal/main.cpp:157).  The blocking schedule
+ * (--noasync) runs the fine propagation as one DoRK4 (Parareal.h:127); the async one, on
+ * ranks r > 0, as one step from tStart and tsF-1 steps from tStartFine = tStart + dtF
+ * (Parareal.h:106,121-123), the receive posted in between. */
 void orc_parareal_parallel(orc_conv* c, int S, int kmax, double T,
                            double dtF, int tsF, double dtG, int tsG, int g_kind,
-                           const double* u0, double* q_iter, double* q_out)
+                           const double* u0, double* q_iter, double* q_out, int async_sched)
 {
     const int ni = c->ni, nj = c->nj, nk = c->nk;
     const long n = (long)ni * nj * nk;
@@ -409,8 +411,15 @@ void orc_parareal_parallel(orc_conv* c, int S, int kmax, double T,
     if (q_iter) orc_field_get(Gold[S - 1], q_iter);
 
     for (int k = 0; k < kmax; ++k) {
-        /* Fine propagation on every rank, Parareal.h:127 */
-        for (int r = 0; r < S; ++r) orc_do_rk4(c, U[r], F[r], slice * r, dtF, tsF);
+        /* Fine propagation on every rank */
+        for (int r = 0; r < S; ++r) {
+            if (async_sched && r > 0) {
+                orc_do_rk4(c, U[r], F[r], slice * r, dtF, 1);
+                orc_do_rk4(c, F[r], F[r], slice * r + dtF, dtF, tsF - 1);
+            } else {
+                orc_do_rk4(c, U[r], F[r], slice * r, dtF, tsF);
+            }
+        }
         /* Receive, coarse propagation, update, send, swap (Parareal.h:131-164) */
         for (int r = 0; r < S; ++r) {
             if (r > 0) orc_field_copy(U[r], q);

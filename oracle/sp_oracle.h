@@ -59,10 +59,13 @@ double orc_computenu(const orc_conv* c, double t);
  * Emulates DoParallel on S ranks in rank order.  u0: compact initial field.
  * q_iter (nullable): (kmax+1) compact fields; entry k is the last rank's q
  * after k iterations (entry 0: the coarse prediction G^{S*tsG}(u0)).
- * q_out: compact final field of the last rank. */
+ * q_out: compact final field of the last rank.
+ * async_sched: 0 = blocking schedule (--noasync); 1 = the async schedule's fine-step
+ * times (Parareal.h:119-124: ranks r > 0 run step 0 at tStart, then tsF-1 steps from
+ * tStart + dtF), which differ from the blocking ones only by rounding of t. */
 void orc_parareal_parallel(orc_conv* c, int S, int kmax, double T,
                            double dtF, int tsF, double dtG, int tsG, int g_kind,
-                           const double* u0, double* q_iter, double* q_out);
+                           const double* u0, double* q_iter, double* q_out, int async_sched);
 /* DoSerial: DoRK4(u0 -> q, 0, dtF, tsF*S) (Parareal.h:250-253) */
 void orc_parareal_serial(orc_conv* c, double dtF, int steps_total,
                          const double* u0, double* q_out);

@@ -547,7 +547,17 @@ int sp_do_rk4_batched(sp_prop* p, int nslices, const sp_layout* lin, const void*
              "sp_do_rk4_batched: dtype pair");
     SP_CHECK(slice_stride >= sp_layout_storage_elems(lin) && slice_stride >= sp_layout_storage_elems(lout),
              SP_ERR_INVALID, "sp_do_rk4_batched: slices overlap");
-    SP_CHECK(!same_origin(lin, in, lout, out), SP_ERR_INVALID, "sp_do_rk4_batched: in and out alias");
+    {
+        // A block reads the columns around its tile while other blocks write theirs: any
+        // overlap of the two batches' storage, not only equal origins, races.
+        const char* ib = (const char*)in;
+        const char* ob = (const char*)out;
+        const long long ie = ((long long)(nslices - 1) * slice_stride + sp_layout_storage_elems(lin)) *
+                             dtype_size(lin->dtype);
+        const long long oe = ((long long)(nslices - 1) * slice_stride + sp_layout_storage_elems(lout)) *
+                             dtype_size(lout->dtype);
+        SP_CHECK(ib + ie <= ob || ob + oe <= ib, SP_ERR_INVALID, "sp_do_rk4_batched: in and out overlap");
+    }
     if (steps == 0) return SP_OK;
     SP_HIP(hipSetDevice(p->ctx->device));
     hipStream_t s = p->ctx->stream;
@@ -1121,8 +1131,9 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
             s = p->ctx->own;
         }
         if (!pr->gexec) {
-            p->ctx->stream = s;
+            // begin the capture first: on failure the context still follows the caller's stream
             SP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            p->ctx->stream = s;
             int r = fine_sweep(pr);
             if (r == SP_OK) r = (pr->cfg.delta_correction ? coarse_chain_delta(pr) : coarse_chain(pr));
             hipGraph_t g = nullptr;
@@ -1165,7 +1176,8 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
         d2 += pr->part_host[i];
         q2 += pr->part_host[i + 1];
     }
-    const double inc = std::sqrt(d2) / std::sqrt(q2);
+    // No change at all (an all-zero solution stays zero) is a zero increment, not 0/0.
+    const double inc = d2 == 0. ? 0. : std::sqrt(d2) / std::sqrt(q2);
     pr->iters_done++;
     if (incr) *incr = inc;
     // A non-finite increment, or slice states whose L2 norm exceeds the initial data's by

@@ -484,7 +484,8 @@ int sp_batch_run(sp_batch* b, void* const* u0, void* const* q, sp_batch_report* 
                 d2 += h[2 * i];
                 q2 += h[2 * i + 1];
             }
-            const double inc = std::sqrt(d2) / std::sqrt(q2);
+            // no change at all (an all-zero problem stays zero) is a zero increment, not 0/0
+            const double inc = d2 == 0. ? 0. : std::sqrt(d2) / std::sqrt(q2);
             SP_CHECK(std::isfinite(inc), SP_ERR_DIVERGED,
                      "sp_batch_run: problem " + std::to_string(p) + " diverged (non-finite increment)");
             b->inc[p].push_back(inc);

@@ -12,6 +12,7 @@
 
 #include "sp/Convection.hpp"
 #include "sp/ConvectionField.hpp"
+#include "sp/ConvectionSolution.hpp"
 #include "sp/MatFile.hpp"
 
 using namespace sp;
@@ -119,21 +120,12 @@ HeatConfiguration parse(int argc, char** argv)
 
 double exactQ(const HeatConfiguration& c, double x, double y, double z, double t)
 {
-    // src/Convection/ConvectionSolution.h:8-20
-    const double pi = 3.14159265358979;
-    const double a = c.nufreq > 0 ? c.nu0 * t - (c.nu0 / (2. * c.nufreq)) * (std::cos(c.nufreq * t) - 1.) : c.nu0 * t;
-    return std::sin(2. * pi * (x - c.cx * t)) * std::sin(2. * pi * (y - c.cy * t)) *
-           std::sin(2. * pi * (z - c.cz * t)) * std::exp(-12. * pi * pi * a);
+    return sp::exactQ(c.nu0, c.nufreq, c.cx, c.cy, c.cz, x, y, z, t);
 }
 
 void fillQ(ConvectionField& q, const HeatConfiguration& c, double t)
 {
-    const IJKSize& d = q.calculationDomain();
-    const double hx = 1. / d.iSize() / 2., hy = 1. / d.jSize() / 2., hz = 1. / d.kSize() / 2.;
-    for (int i = 0; i < d.iSize(); ++i)
-        for (int j = 0; j < d.jSize(); ++j)
-            for (int k = 0; k < d.kSize(); ++k)
-                q(i, j, k) = exactQ(c, (2 * i + 1) * hx, (2 * j + 1) * hy, (2 * k + 1) * hz, t);
+    sp::fillQ(q, c.nu0, c.nufreq, c.cx, c.cy, c.cz, t, 0., 1., 0., 1., 0., 1.);
 }
 
 // computeError, src/Convection/ConvectionSolution.h:55-103
@@ -162,26 +154,13 @@ int main(int argc, char** argv)
 {
     const HeatConfiguration conf = parse(argc, argv);
     const double h = 1. / conf.gridsize;
-    std::cout << "Running with:\n"
-              << " - grid size: " << conf.gridsize << "x" << conf.gridsize << "x" << conf.gridsize << "\n"
-              << " - spatial discretization step: " << conf.dx << "\n"
-              << " - timestep size: " << conf.dt << "\n"
-              << " - timesteps: " << conf.timesteps << "\n"
-              << " - endtime: " << conf.endtime << "\n"
-              << " - initial diffusion coefficient: " << conf.nu0 << "\n"
-              << " - frequency of diffusion coefficient: " << conf.nufreq << "\n"
-              << " - advection velocity in x: " << conf.cx << "\n"
-              << " - advection velocity in y: " << conf.cy << "\n"
-              << " - advection velocity in z: " << conf.cz << "\n"
-              << " - CFL: " << conf.cfl << "\n"
-              << " - solver: " << (conf.rk ? "Runge-Kutta 4" : "explicit Euler") << "\n"
-              << "\n"
-              << "Spatial configuration:\n"
-              << " - Process 0 is (0, 0, 0)\n"
-              << "   -- Local grid: " << conf.gridsize << "x" << conf.gridsize << "x" << conf.gridsize << "\n"
-              << "   -- Starting coordinates: (0, 0, 0)\n"
-              << "   -- Ending coordinates: (" << conf.gridsize * h << ", " << conf.gridsize * h << ", "
-              << conf.gridsize * h << ")\n";
+    std::cout << "convection: 3-D advection-diffusion on the periodic unit cube, one GPU\n"
+              << "  grid " << conf.gridsize << "^3 (dx = " << conf.dx << "), " << conf.timesteps
+              << " steps of dt = " << conf.dt << " to T = " << conf.endtime << ", CFL " << conf.cfl << "\n"
+              << "  nu0 = " << conf.nu0 << ", nufreq = " << conf.nufreq << ", c = (" << conf.cx << ", " << conf.cy
+              << ", " << conf.cz << ")\n"
+              << "  solver: " << (conf.rk ? "Runge-Kutta 4" : "explicit Euler") << "\n"
+              << "  domain [0, " << conf.gridsize * h << "]^3 on one process\n\n";
 
     try {
         IJKSize domain;

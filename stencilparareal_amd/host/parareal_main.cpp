@@ -15,6 +15,7 @@
 
 #include "sp/Convection.hpp"
 #include "sp/ConvectionField.hpp"
+#include "sp/ConvectionSolution.hpp"
 #include "sp/MatFile.hpp"
 #include "sp/Parareal.hpp"
 #include "sp/RuntimeConfiguration.hpp"
@@ -26,25 +27,10 @@ static double wtime()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// exactQ / fillQ, src/Convection/ConvectionSolution.h:8-52 (unit cube, t = 0 here)
-static double exactQ(double nu0, double nufreq, double cx, double cy, double cz, double x, double y, double z,
-                     double t)
-{
-    const double pi = 3.14159265358979;
-    const double a = nufreq > 0 ? nu0 * t - (nu0 / (2. * nufreq)) * (std::cos(nufreq * t) - 1.) : nu0 * t;
-    return std::sin(2. * pi * (x - cx * t)) * std::sin(2. * pi * (y - cy * t)) * std::sin(2. * pi * (z - cz * t)) *
-           std::exp(-12. * pi * pi * a);
-}
-
+// fillQ at t with the run's coefficients on the unit cube (ConvectionSolution.h:23-52)
 static void fillQ(ConvectionField& q, const RuntimeConfiguration& conf, double t)
 {
-    const IJKSize& d = q.calculationDomain();
-    const double dxh = 1. / d.iSize() / 2., dyh = 1. / d.jSize() / 2., dzh = 1. / d.kSize() / 2.;
-    for (int i = 0; i < d.iSize(); ++i)
-        for (int j = 0; j < d.jSize(); ++j)
-            for (int k = 0; k < d.kSize(); ++k)
-                q(i, j, k) = exactQ(conf.nu0(), conf.nufreq(), conf.cx(), conf.cy(), conf.cz(), (2 * i + 1) * dxh,
-                                    (2 * j + 1) * dyh, (2 * k + 1) * dzh, t);
+    sp::fillQ(q, conf.nu0(), conf.nufreq(), conf.cx(), conf.cy(), conf.cz(), t, 0., 1., 0., 1., 0., 1.);
 }
 
 // computeErrorReference, src/Parareal/main.cpp:98-139
@@ -79,25 +65,18 @@ int main(int argc, char** argv)
         if (k == "--autotune") opt.autotune = v != "0";
     }
 
-    std::cout << "Running with:\n"
-              << " - initial diffusion coefficient: " << conf.nu0() << "\n"
-              << " - frequence of diffusion coefficient: " << conf.nufreq() << "\n"
-              << " - advection velocity in x: " << conf.cx() << "\n"
-              << " - advection velocity in y: " << conf.cy() << "\n"
-              << " - advection velocity in z: " << conf.cz() << "\n"
-              << " - spatial discretization step: " << conf.dx() << "\n"
-              << " - endtime: " << conf.endTime() << "\n"
-              << " - number of time slices: " << conf.timeSlices() << "\n"
-              << " - time slice size: " << conf.timeSliceSize() << "\n"
-              << " - CFL fine: " << conf.cflFine() << "\n"
-              << " - CFL coarse: " << conf.cflCoarse() << "\n"
-              << " - timestep size fine: " << conf.dtFine() << "\n"
-              << " - timestep size coarse: " << conf.dtCoarse() << "\n"
-              << " - timesteps per slice fine propagator: " << conf.timeStepsFinePerTimeSlice() << "\n"
-              << " - timesteps per slice coarse propagator: " << conf.timeStepsCoarsePerTimeSlice() << "\n"
-              << " - parareal iterations: " << conf.kmax() << "\n"
-              << " - asynchronous communications: " << (conf.async() ? "Enabled" : "Disabled") << "\n"
-              << " - intermediate fields in mat files: " << (conf.mat() ? "Yes" : "No") << "\n"
+    std::cout << "parareal: time-parallel advection-diffusion, all " << conf.timeSlices()
+              << " time slices batched on one GPU\n"
+              << "  grid " << conf.gridSize() << "^3 (dx = " << conf.dx() << "), T = " << conf.endTime()
+              << ", slice length " << conf.timeSliceSize() << "\n"
+              << "  nu0 = " << conf.nu0() << ", nufreq = " << conf.nufreq() << ", c = (" << conf.cx() << ", "
+              << conf.cy() << ", " << conf.cz() << ")\n"
+              << "  fine RK4:  dt = " << conf.dtFine() << ", " << conf.timeStepsFinePerTimeSlice()
+              << " steps per slice, CFL " << conf.cflFine() << "\n"
+              << "  coarse:    dt = " << conf.dtCoarse() << ", " << conf.timeStepsCoarsePerTimeSlice()
+              << " steps per slice, CFL " << conf.cflCoarse() << "\n"
+              << "  iterations " << conf.kmax() << ", async fine-step times " << (conf.async() ? "on" : "off")
+              << ", mat output " << (conf.mat() ? "on" : "off") << "\n"
               << std::endl;
 
     try {
@@ -110,7 +89,8 @@ int main(int argc, char** argv)
         qinitial.Init("qinitial", domain, kboundary);
         Convection convection(conf.gridSize(), conf.gridSize(), conf.gridSize(), conf.dx(), conf.nu0(),
                               conf.nufreq(), conf.cx(), conf.cy(), conf.cz());
-        Parareal<Convection, ConvectionField> parareal(convection, qinitial, q, conf, opt);
+        // the reference's six-argument form: this process is rank 0 of one, its slice starts at 0
+        Parareal<Convection, ConvectionField> parareal(convection, qinitial, q, 0., conf, COMM_WORLD, opt);
 
         if (conf.mode() == ModeCompare) {
             // cost ratio of the propagators on 4 slices' steps (main.cpp:197-210)

@@ -292,6 +292,29 @@ def test_initial_conditions(sp):
     assert np.abs(f.get() - O.fill_gaussians(N, *args)).max() < 1e-14
 
 
+def test_batched_rk4_rejects_overlapping_batches(sp):
+    # in and out batches that overlap without sharing an origin race (blocks read their
+    # tile's halo while others write): refused before any launch
+    from stencilparareal_amd import _lib as L
+    import ctypes as C
+    N, S = 8, 3
+    conv = sp.Convection(N, N, N, 1. / N, 0.01, 0.0, *C3)
+    lay = conv.field().layout
+    n = N ** 3
+    buf = torch.zeros((S + 1) * n, dtype=torch.float64, device="cuda")
+    base = buf.data_ptr()
+    for off in (n, n // 2, 8):  # out = in + one slice, half a slice, one row
+        with pytest.raises(L.SPError) as ei:
+            L.call("sp_do_rk4_batched", conv.h, S, C.byref(lay), C.c_void_p(base), C.byref(lay),
+                   C.c_void_p(base + off * 8), n, 0.0, 1e-4, 1)
+        assert ei.value.code == L.SP_ERR_INVALID
+    # disjoint batches (out right after in) are accepted
+    buf2 = torch.zeros(2 * S * n, dtype=torch.float64, device="cuda")
+    L.call("sp_do_rk4_batched", conv.h, S, C.byref(lay), C.c_void_p(buf2.data_ptr()), C.byref(lay),
+           C.c_void_p(buf2.data_ptr() + S * n * 8), n, 0.0, 1e-4, 1)
+    torch.cuda.synchronize()
+
+
 def test_invalid_arguments_return_errors(sp):
     from stencilparareal_amd._lib import SPError
     conv = sp.Convection(8, 8, 8, 1. / 8, 0.01, 0.0, *C3)
@@ -309,15 +332,15 @@ def test_invalid_arguments_return_errors(sp):
 # ---------------------------------------------------------------------------------------------
 # Parareal
 
-def _parareal_case(sp, path, coarse, N=12, S=4, kmax=4, nufreq=0.0):
+def _parareal_case(sp, path, coarse, N=12, S=4, kmax=4, nufreq=0.0, async_=False):
     conf = RuntimeConfiguration(gridSize=N, endTime=0.02, timeStepsFine=64, timeStepsCoarse=8, nu0=0.05,
-                                nufreq=nufreq, cx=1.0, cy=-0.5, cz=0.25, kmax=kmax, async_=False,
+                                nufreq=nufreq, cx=1.0, cy=-0.5, cz=0.25, kmax=kmax, async_=async_,
                                 timeSlices=S)
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, path=path)
     u0 = O.fill_gaussians(N, [0.3, 0.6], [0.4, 0.7], [0.5, 0.2], [0.1, 0.08], [1.0, 0.5])
     oc = O.OConvection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz)
     _, iters = oc.parareal(u0, S, kmax, conf.endTime, conf.dtFine(), conf.timeStepsFinePerTimeSlice(),
-                           conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(), coarse)
+                           conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(), coarse, async_=async_)
     serial = oc.serial(u0, conf.dtFine(), conf.timeStepsFine)
     return conf, conv, u0, iters, serial
 
@@ -334,6 +357,45 @@ def test_parareal_staged_bitwise_every_iterate(sp, coarse, nufreq):
     for k in range(1, conf.kmax + 1):
         pr.iterate()
         assert np.array_equal(pr.solution().get(), iters[k]), k
+
+
+@pytest.mark.parametrize("coarse", ["euler", "rk4"])
+def test_parareal_async_schedule_staged_bitwise(sp, coarse):
+    # the async schedule's fine-step times (Parareal.h:119-124: ranks r > 0 run step 0 at
+    # tStart, then from tStart + dtF), with a time-dependent nu so the times matter
+    conf, conv, u0, iters, _ = _parareal_case(sp, "staged", coarse, nufreq=80.0, async_=True)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    pr = sp.Parareal(conv, init, sol, conf, coarse=coarse)
+    assert pr.cfg.async_schedule == 1
+    pr.begin()
+    assert np.array_equal(pr.solution().get(), iters[0])
+    for k in range(1, conf.kmax + 1):
+        pr.iterate()
+        assert np.array_equal(pr.solution().get(), iters[k]), k
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_parareal_async_schedule_fused_vs_oracle(sp, graphs):
+    conf, conv, u0, iters, _ = _parareal_case(sp, "fused", "rk4", nufreq=80.0, async_=True)
+    init, sol = conv.field(), conv.field()
+    init.set(u0)
+    rep = sp.Parareal(conv, init, sol, conf, coarse="rk4", use_graphs=graphs).DoParallel()
+    assert rep.iterations == conf.kmax
+    assert relerr(sol.get(), iters[conf.kmax]) <= 1e-12
+
+
+@pytest.mark.parametrize("gdt", ["f64", "f32"])
+def test_parareal_zero_initial_field(sp, gdt):
+    # an all-zero field stays zero: every increment is 0 (not 0/0), the solution is zero
+    conf, conv, _, _, _ = _parareal_case(sp, "fused", "rk4", kmax=2)
+    init, sol = conv.field(), conv.field()
+    init.set(np.zeros(init.shape))
+    sol.set(np.ones(sol.shape))
+    rep = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=gdt).DoParallel()
+    assert rep.iterations == 2
+    assert list(rep.increment[:2]) == [0.0, 0.0]
+    assert not np.any(sol.get())
 
 
 @pytest.mark.parametrize("coarse", ["euler", "rk4"])

@@ -251,3 +251,29 @@ def test_parareal_first_iterate_by_hand(oracle_lib):
     _, it = conv.parareal(u0, S, 1, T, dtF, tsF, dtG, tsG, "euler")
     assert np.array_equal(it[1], q)
     assert np.array_equal(it[0], Gold[S - 1])
+
+
+def test_parareal_async_schedule_times(oracle_lib):
+    # The async schedule (Parareal.h:119-124) changes only the fine-step times of ranks
+    # r > 0: step 0 at tStart, steps i >= 1 at (tStart + dtF) + (i-1)*dtF instead of
+    # tStart + i*dtF.  With constant nu the times never enter: bit-for-bit the blocking
+    # result.  With nu(t) the two differ by rounding of t only, far below what a shifted or
+    # skipped step would change.
+    N, S, T, tsF, tsG = 8, 4, 0.02, 8, 2
+    dtF, dtG = T / (S * tsF), T / (S * tsG)
+    u0 = O.fill_exact(N, 0.1, 0.0, 1.0, 1.0, 1.0, 0.0)
+    c0 = O.OConvection(N, N, N, 1. / N, 0.05, 0.0, 1.0, -0.5, 0.25)
+    _, blk = c0.parareal(u0, S, 2, T, dtF, tsF, dtG, tsG, "rk4")
+    _, asy = c0.parareal(u0, S, 2, T, dtF, tsF, dtG, tsG, "rk4", async_=True)
+    assert np.array_equal(blk, asy)
+    c1 = O.OConvection(N, N, N, 1. / N, 0.05, 80.0, 1.0, -0.5, 0.25)
+    _, blk = c1.parareal(u0, S, 2, T, dtF, tsF, dtG, tsG, "rk4")
+    _, asy = c1.parareal(u0, S, 2, T, dtF, tsF, dtG, tsG, "rk4", async_=True)
+    assert O.error_reference(asy[2], blk[2]) < 1e-13
+    # a plausible mistake: the second DoRK4 started one step late (tStart + 2 dtF)
+    sl = T / S
+    U1 = c1.rk4(u0, 0.0, dtG, tsG)  # a start state for rank 1
+    good = c1.rk4(c1.rk4(U1, sl, dtF, 1), sl + dtF, dtF, tsF - 1)
+    late = c1.rk4(c1.rk4(U1, sl, dtF, 1), sl + 2 * dtF, dtF, tsF - 1)
+    assert O.error_reference(late, good) > 1e-9
+    assert O.error_reference(good, c1.rk4(U1, sl, dtF, tsF)) < 1e-13
