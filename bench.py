@@ -1,30 +1,40 @@
 #!/usr/bin/env python3
-"""Benchmark: batched Parareal on one MI355X for the 128^3 / 16-slice FP64 workload.
+"""Benchmark: batched mixed-precision Parareal on one MI355X, 384^3 grid with 64 time slices.
 
-Workload (the configuration the metric is quoted on): 3-D advection-diffusion on the
-periodic unit cube, 128^3 grid, initial field = 8 Gaussians (seeded SplitMix64 draws),
-c = (1, 0.5, 0.25), nu = 5e-3, T = 1, 16 time slices batched on one GPU, fine RK4
-dt = 5e-5 (20000 steps), coarse RK4 dt = 1/1008 (1000 steps rounded up to a multiple
-of 16), K = 5 Parareal iterations, everything FP64.
+Workload (the largest configuration; one GPU holds it): 3-D advection-diffusion on the
+periodic unit cube, 384^3 grid, initial field = 8 Gaussians (SplitMix64 seed 0), c uniform
+in [-1, 1]^3 (seed 1), nu = 1e-3, T = 2, 64 time slices batched on one GPU.  F: FP64 RK4,
+dt = 2e-5 (100000 steps rounded up to 100032, 1563 per slice).  G: RK4 with BF16 storage
+and FP32 arithmetic, coarse step 2/1024 split into 2 sub-steps (the whole step is unstable
+on this grid), difference-form correction in FP64.  Stop once the relative L2 increment of
+the slice states is <= 1e-6 (at most 8 iterations).
 
-One "step" is one full Parareal solve (initial coarse sweep + K iterations) of that
-workload with its input already resident in device memory.  `value` is the stencil
-throughput of the solve: lattice-point updates (fine RK4 steps + coarse RK4 steps) per
-second.  The line also carries the serial-fine wall time on the same GPU, the Parareal
-speedup over it, the error of each iterate against the serial FP64 fine solution, the
-wall time to 1e-6 of batched Parareal with FP64 and FP32 coarse propagators against plain
-all-FP64 Parareal in the reference's execution structure (reference-order sweeps, slice
-after slice) on the same GPU, the roofline of the dominant kernel (the batched fused RK4
-step), and the CPU baseline (the oracle's serial fine sweep on the host cores, a bounded
-sample).
+One "step" is one batched fused RK4 fine step over all 64 slices (64 x 384^3 lattice
+updates in FP64, input and output resident in HBM): the loop that makes up ~97 % of every
+Parareal iteration.  `value` = lattice updates per second of the timed steps; the roofline
+is that kernel's, timed with HIP events on its launch stream, with the fraction of the
+measured copy bandwidth beside the fraction of HBM peak.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+Timed once beside the steps (not part of `value`):
+  * the whole Parareal solve stopped by its 1e-6 increment test, with the fine sweep and
+    the coarse chain of every iteration timed by HIP events;
+  * the serial FP64 fine solve on the same GPU (also the error reference), and the
+    Parareal speedup over it;
+  * plain all-FP64 Parareal in the reference's execution structure (reference-order sweeps,
+    slice after slice, FP64 G, host launches), extrapolated from measured step times;
+  * the CPU baseline: the oracle on the host cores (child process, threads bound to
+    physical cores), a bounded sample of the serial fine sweep on the same grid, and the
+    32^3 / 4-slice CPU configuration's Parareal in full.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-solve] [--no-cpu]
+                    [--workload cfg4|cfg2|cfg5] [--roofline-only]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -32,96 +42,93 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "Parareal wall-time vs serial-FP64-fine at ≤1e-6 rel L2; stencil GLUP/s + %HBM peak"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, budget_s=12.0):
-    """The oracle's serial fine RK4 (reference OpenMP structure) on the host, bounded."""
-    import oracle as O
-    from stencilparareal_amd.config import rounded_steps
-    N = w.N
-    g = w.gauss
-    q0 = O.fill_gaussians(N, g.xc, g.yc, g.zc, g.sigma, g.amp)
-    conv = O.OConvection(N, N, N, 1.0 / N, w.nu, 0.0, *w.c)
-    dt = w.T / rounded_steps(w.stepsF, w.slices)
-    t0 = O.wtime()
-    conv.rk4(q0, 0.0, dt, 2)
-    t_probe = max(O.wtime() - t0, 1e-3)
-    n = max(2, min(20000, int(budget_s / (t_probe / 2))))
-    t0 = O.wtime()
-    conv.rk4(q0, 0.0, dt, n)
-    t = O.wtime() - t0
-    threads = O.max_threads()
-    # the same sweep on one thread, a shorter sample
-    O.set_threads(1)
-    try:
-        n1 = max(2, int(3.0 / (t / n * threads)))
-        t1 = O.wtime()
-        conv.rk4(q0, 0.0, dt, n1)
-        t1 = O.wtime() - t1
-    finally:
-        O.set_threads(threads)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
-    return {"value": round(N ** 3 * n / t / 1e9, 5), "unit": "GLUP/s", "cores": threads,
-            "kind": "port",
-            "sample": f"serial fine RK4 (oracle, reference OpenMP structure), {N}^3, {n} steps, {t:.2f} s",
-            "one_thread": {"value": round(N ** 3 * n1 / t1 / 1e9, 5), "sample": f"{n1} steps, {t1:.2f} s"},
-            "nproc": os.cpu_count(), "cpu_model": model}
+# ---------------------------------------------------------------------------------------
+# pieces shared by the workloads
+
+def conf_of(w, **kw):
+    from stencilparareal_amd.config import RuntimeConfiguration
+    d = dict(gridSize=w.N, endTime=w.T, timeStepsFine=w.stepsF, timeStepsCoarse=w.stepsG, nu0=w.nu,
+             cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=w.kmax, async_=False, timeSlices=w.slices)
+    d.update(kw)
+    return RuntimeConfiguration(**d)
 
 
-def measure_batched_step(conv, init, conf, S, reps=50, copy_check=False):
-    """Average duration of one batched fused RK4 step over S slices (sp_do_rk4_batched),
-    timed with HIP events on the stream the library launches on (torch's current stream,
-    which the context follows).  Returns (ms, algorithmic bytes)."""
-    import torch
-    from stencilparareal_amd import _lib as L
-    N = init.layout.ni
-    lay = init.layout
-    n = N ** 3
-    bin_ = torch.empty(S * n, dtype=torch.float64, device="cuda")
-    bout = torch.empty_like(bin_)
-    for b in range(S):
-        bin_[b * n:(b + 1) * n].copy_(init.interior().reshape(-1))
-    conv.ctx.sync_stream()
-    dt_f = conf.dtFine()
+class Batch2:
+    """Two device arrays of S slices (ping-pong) and the batched fused fine step over them."""
 
-    def steps(src, dst, k):
-        # one library call, k launches queued back to back (ping-pong with the library's
-        # batch scratch), so the events see kernel time rather than host-side call gaps
-        L.call("sp_do_rk4_batched", conv.h, S, C_ptr(lay), ctypes_vp(src), C_ptr(lay), ctypes_vp(dst), n, 0.0,
-               dt_f, k)
+    def __init__(self, conv, init, S):
+        import torch
+        self.conv, self.S, self.lay = conv, S, init.layout
+        self.n = init.layout.ni * init.layout.nj * init.layout.nk
+        self.a = torch.empty(S * self.n, dtype=torch.float64, device="cuda")
+        self.b = torch.empty_like(self.a)
+        src = init.interior().reshape(-1)
+        for s in range(S):
+            self.a[s * self.n:(s + 1) * self.n].copy_(src)
+        conv.ctx.sync_stream()
 
-    steps(bin_, bout, 3)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    steps(bin_, bout, reps)
-    ev1.record()
-    ev1.synchronize()
-    k_ms = ev0.elapsed_time(ev1) / reps
-    if copy_check:
-        # a plain streaming kernel over the same bytes (read S*N^3 doubles, write as many):
-        # the scale for the PMC byte counters
+    def step(self, src, dst, dt):
+        import ctypes as C
+        from stencilparareal_amd import _lib as L
+        L.call("sp_do_rk4_batched", self.conv.h, self.S, C.byref(self.lay), C.c_void_p(src.data_ptr()),
+               C.byref(self.lay), C.c_void_p(dst.data_ptr()), self.n, 0.0, dt, 1)
+
+    def run(self, k, dt, events=None):
+        """k steps, each one launch of the fused kernel, alternating a -> b -> a."""
+        import torch
+        if events:
+            events[0].record()
+        for i in range(k):
+            src, dst = (self.a, self.b) if i % 2 == 0 else (self.b, self.a)
+            self.step(src, dst, dt)
+        if events:
+            events[1].record()
+        return torch
+
+    def copy_gbs(self, reps=10):
+        """Bandwidth of a plain device copy of the same bytes (read S*n doubles, write as many)."""
+        import torch
+        self.b.copy_(self.a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         for _ in range(reps):
-            torch.mul(bin_, 1.0, out=bout)
-        torch.cuda.synchronize()
-    return k_ms, 16.0 * S * n  # FP64 state read once + written once per lattice update
+            self.b.copy_(self.a)
+        e1.record()
+        e1.synchronize()
+        return 16.0 * self.S * self.n * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+
+    def free(self):
+        import torch
+        del self.a, self.b
+        torch.cuda.empty_cache()
 
 
-def committed_traffic(variant):
-    """HBM bytes per launch of the batched step with the given fused-kernel variant, from
-    the committed PMC summary (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled
-    by a plain copy kernel); None when no summary exists for that variant."""
-    path = os.path.join(ROOT, "profiles", f"pmc_rk4_fused_{variant}_16x128cubed_fp64.txt")
+def timed_steps(bt, a, dt):
+    """The timed region: W untimed steps, then K steps between device synchronisations;
+    wall clock for ms_per_step, HIP events on the launch stream for the kernel average."""
+    import torch
+    bt.run(a.warmup, dt)
+    torch.cuda.synchronize()
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    t0 = time.perf_counter()
+    bt.run(a.steps, dt, ev)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / a.steps * 1e3, ev[0].elapsed_time(ev[1]) / a.steps
+
+
+def committed_traffic(name):
+    """HBM bytes per launch of the dominant kernel from the committed PMC summary
+    profiles/<name> (FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel); None if absent."""
     try:
-        for line in open(path):
+        for line in open(os.path.join(ROOT, "profiles", name)):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":")[1])
     except OSError:
@@ -129,38 +136,252 @@ def committed_traffic(variant):
     return None
 
 
-def roofline_only(a):
-    """Only the dominant kernel (and the copy kernel used as the PMC scale), for profilers."""
+def roofline(k_ms, alg_bytes, copy_gbs, traffic, kernel):
+    achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    actual = traffic / (k_ms / 1e3) / 1e9 if traffic else None
+    # what bounds the kernel: HBM once the counted traffic streams near the copy rate,
+    # otherwise the on-chip work (LDS and FP64 VALU issue of the fused stages)
+    limiter = None
+    if actual:
+        limiter = "hbm" if actual >= 0.85 * copy_gbs else "on-chip (LDS + FP64 VALU issue, latency)"
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "frac_of_copy": round(achieved / copy_gbs, 4), "copy_gbs": round(copy_gbs, 1),
+            "traffic_rate_gbs": round(actual, 1) if actual else None, "limiter": limiter,
+            "kernel": kernel, "avg_launch_ms": round(k_ms, 4), "bytes_per_launch": alg_bytes}
+
+
+def cpu_baseline(N, nu, c, dt, gauss_cfg, budget=12.0):
+    """The oracle's serial fine RK4 (the reference's OpenMP structure) on the host cores, a
+    bounded sample on the benchmark grid, plus the CPU configuration's Parareal in full; run
+    as a child process with OMP_NUM_THREADS = physical cores, OMP_PROC_BIND=close."""
+    from oracle.cpu_bench import child_env
+    env, phys, logical = child_env()
+    cmd = [sys.executable, "-m", "oracle.cpu_bench", "--grid", str(N), "--nu", repr(nu), "--c",
+           *[repr(x) for x in c], "--dt", repr(dt), "--gauss-cfg", str(gauss_cfg), "--budget", str(budget)]
+    r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log("cpu baseline failed: " + r.stderr[-500:])
+        return None
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    sf = d["serial_fine"]
+    return {"value": sf["glups"], "unit": "GLUP/s", "cores": sf["threads"], "kind": "port",
+            "sample": f"serial fine RK4 (oracle, reference OpenMP structure), {N}^3, {sf['steps']} steps, "
+                      f"{sf['seconds']} s, OMP_PROC_BIND=close on {phys} physical cores",
+            "one_thread": d["one_thread"], "parareal_cfg1": d.get("parareal_cfg1"),
+            "physical_cores": phys, "logical_cpus": logical, "cpu_model": model}
+
+
+def plain_step_ms(w, dt, steps=10, gdt=None):
+    """One slice's reference-structure RK4 step (4 RHS + 3 Euler + 1 RK sweep, host-launched),
+    timed with HIP events."""
     import torch
     import stencilparareal_amd as sp
-    from stencilparareal_amd.config import RuntimeConfiguration, workload
+    conv = sp.Convection(w.N, w.N, w.N, 1.0 / w.N, w.nu, 0.0, *w.c, path="staged")
+    a, b = conv.field("pa"), conv.field("pb")
+    g = w.gauss
+    sp.fill_gaussians(a, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    conv.DoRK4(a, b, 0.0, dt, 2)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    conv.DoRK4(a, b, 0.0, dt, steps)
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+# ---------------------------------------------------------------------------------------
+# the default workload: 384^3, 64 slices, BF16-storage G, stop at 1e-6
+
+def cfg4(a):
+    import torch
+    import stencilparareal_amd as sp
+    from stencilparareal_amd.config import workload
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    w = workload(2)
+    w = workload(4)
     N, S = w.N, w.slices
-    conf = RuntimeConfiguration(gridSize=N, endTime=w.T, timeStepsFine=w.stepsF, timeStepsCoarse=w.stepsG,
-                                nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=w.kmax, async_=False,
-                                timeSlices=S)
+    conf = conf_of(w)
+    tsF, dtF = conf.timeStepsFinePerTimeSlice(), conf.dtFine()
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
-    if a.variant == "auto":
-        conv.autotune(S)  # the tile the benchmark's Parareal run picks
-    else:
-        # a fixed kernel: under counter collection the autotuner's timings are not the
-        # kernels' own, so profiles name the kernel they measure
-        conv.set_tile(32, 32, 512, 0, variant=a.variant)
     init = conv.field("qinitial")
     g = w.gauss
     sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
-    k_ms, alg = measure_batched_step(conv, init, conf, S, reps=a.reps, copy_check=True)
-    print(json.dumps({"kernel_avg_ms": round(k_ms, 4), "bytes_per_launch": alg, "tile": conv.get_tile(S, True),
-                      "copy_bytes_read": alg / 2, "copy_bytes_written": alg / 2, "reps": a.reps}), flush=True)
+    log("autotune the fused tile for 64 slices")
+    conv.autotune(S)
+    tile = conv.get_tile(S, True)
+    bt = Batch2(conv, init, S)
+    log(f"warmup {a.warmup}, timed {a.steps} batched steps")
+    ms, k_ms = timed_steps(bt, a, dtF)
+    copy = bt.copy_gbs()
+    bt.free()
+    lup = S * N ** 3
+    kernel = (f"k_rk4_fused_{tile['variant']} {tile['tx']}x{tile['ty']} tiles, {tile['nthreads']} threads "
+              f"(FP64, {S} slices of {N}^3, kchunk {tile['kchunk']})")
+    rl = roofline(k_ms, 16.0 * lup, copy,
+                  committed_traffic(f"pmc_rk4_fused_{tile['variant']}_64x384cubed_fp64.txt"), kernel)
+    out = {
+        "metric": METRIC, "value": round(lup / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (8 seeded Gaussians, SplitMix64 seed 0; velocity seed 1)",
+        "config": {"workload": "384^3 grid, 64 time slices batched on one GPU, F FP64 RK4 dt=2e-5 (1563 steps "
+                               "per slice), G RK4 BF16 storage / FP32 arithmetic dt=2/1024 in 2 sub-steps, "
+                               "difference-form correction, stop at rel. increment 1e-6 (K<=8), T=2, nu=1e-3",
+                   "grid": N, "slices": S, "c": [round(x, 6) for x in w.c],
+                   "step": "one batched fused RK4 fine step over all 64 slices (64 x 384^3 lattice updates)",
+                   "parallelism": "slices-as-batch"},
+        "roofline": rl,
+    }
+    if not a.no_solve:
+        out.update(cfg4_solve(a, w, conf, conv, init))
+    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(N, w.nu, w.c, dtF, 4)
+    print(json.dumps(out), flush=True)
 
 
-def uneven(a):
-    """The uneven batch: 24 independent problems (64^3 .. 160^3, 8 .. 24 slices, nu and c
-    drawn from seed 0) in one Parareal batch, F in FP64, G in FP32 (difference-form
-    correction), tiles autotuned per grid size, K = 5.  value = lattice updates of the
-    solves (fine + coarse) per second; per-problem error against its serial FP64 fine
-    solution on the same GPU."""
+def cfg4_solve(a, w, conf, conv, init):
+    """The once-timed runs beside the steps: serial fine, the tolerance-stopped Parareal solve
+    and the plain comparator estimate."""
+    import torch
+    import stencilparareal_amd as sp
+    N, S = w.N, w.slices
+    tsF, dtF = conf.timeStepsFinePerTimeSlice(), conf.dtFine()
+    sol, serial = conv.field("q"), conv.field("qreference")
+    log("create Parareal (BF16 G, 2 sub-steps, tol 1e-6, graphs; G tile autotuned)")
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=w.coarse_dtype, substeps=w.substeps,
+                     tol=w.tol, use_graphs=True, autotune=True)
+    log("serial FP64 fine solve (DoSerial)")
+    pr.q = serial
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.DoSerial()
+    torch.cuda.synchronize()
+    t_serial = time.perf_counter() - t0
+    pr.q = sol
+    # DoParallel (Parareal.h:110-171) step by step, so each iteration logs a line: begin
+    # (initial coarse chain), iterations until the increment test, the last slice's state
+    log("Parareal solve to 1e-6")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.begin()
+    t_init = time.perf_counter() - t0
+    iters = []
+    for k in range(w.kmax):
+        inc = pr.iterate()
+        f_s, c_s, _ = pr.iteration_times()
+        iters.append({"increment": float(f"{inc:.3e}"), "fine_s": round(f_s, 4), "chain_s": round(c_s, 4)})
+        log(f"iteration {k + 1}: increment {inc:.3e}, fine sweep {f_s:.2f} s, coarse chain {c_s:.3f} s")
+        if inc <= w.tol:
+            break
+    pr.solution(sol)
+    torch.cuda.synchronize()
+    t_par = time.perf_counter() - t0
+    K = len(iters)
+    err_l2, err_inf = sp.rel_l2(sol, serial), sp.error_reference(sol, serial)
+    tsG = conf_of(w).timeStepsCoarsePerTimeSlice() * w.substeps
+    dtG = conf.dtCoarse() / w.substeps
+    del pr
+    # plain all-FP64 Parareal in the reference's execution structure, from measured step
+    # times: K iterations of S slices x tsF reference-order fine steps, S-1 coarse segments
+    # of tsG FP64 reference-order RK4 sub-steps, plus the initial coarse chain (S segments)
+    log("plain comparator: reference-order step times")
+    f_ms = plain_step_ms(w, dtF)
+    g_ms = plain_step_ms(w, dtG)
+    plain_s = (K * (S * tsF * f_ms + (S - 1) * tsG * g_ms) + S * tsG * g_ms) / 1e3
+    fine_sum = sum(x["fine_s"] for x in iters)
+    chain_sum = sum(x["chain_s"] for x in iters)
+    return {
+        "parareal": {"wall_s": round(t_par, 3), "iterations": K, "tol": w.tol, "init_coarse_s": round(t_init, 4),
+                     "per_iteration": iters,
+                     "iteration_over_fine_sweep": round((fine_sum + chain_sum) / fine_sum, 4),
+                     "rel_l2_err": float(f"{err_l2:.3e}"), "rel_linf_err": float(f"{err_inf:.3e}"),
+                     "fine_glups_in_solve": round(K * S * tsF * N ** 3 / fine_sum / 1e9, 3)},
+        "serial_fine": {"wall_s": round(t_serial, 3), "steps": S * tsF,
+                        "glups": round(S * tsF * N ** 3 / t_serial / 1e9, 3)},
+        "speedup_vs_serial_fine": round(t_serial / t_par, 4),
+        "plain_fp64_parareal_reference_structure": {
+            "wall_s_extrapolated": round(plain_s, 1), "iterations_assumed": K,
+            "fine_step_ms": round(f_ms, 4), "coarse_substep_ms": round(g_ms, 4),
+            "how": "reference-order sweeps (4 RHS + 3 Euler + 1 RK kernels per step), one slice after another, "
+                   "FP64 G; measured step times x the solve's step counts, same iteration count"},
+        "speedup_vs_plain_fp64_parareal": round(plain_s / t_par, 2),
+    }
+
+
+# ---------------------------------------------------------------------------------------
+# the 128^3 / 16-slice FP64 workload (configuration 2)
+
+def cfg2(a):
+    import torch
+    import stencilparareal_amd as sp
+    from stencilparareal_amd.config import workload
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    w = workload(2)
+    N, S = w.N, w.slices
+    conf = conf_of(w)
+    tsF = conf.timeStepsFinePerTimeSlice()
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
+    init, sol, serial = conv.field("qinitial"), conv.field("q"), conv.field("qreference")
+    g = w.gauss
+    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", use_graphs=True, autotune=True)
+    tile = conv.get_tile(S, True)
+    bt = Batch2(conv, init, S)
+    ms, k_ms = timed_steps(bt, a, conf.dtFine())
+    copy = bt.copy_gbs()
+    bt.free()
+    lup = S * N ** 3
+    kernel = (f"k_rk4_fused_{tile['variant']} {tile['tx']}x{tile['ty']} tiles, {tile['nthreads']} threads "
+              f"(FP64, {S} slices of {N}^3, kchunk {tile['kchunk']})")
+    rl = roofline(k_ms, 16.0 * lup, copy,
+                  committed_traffic(f"pmc_rk4_fused_{tile['variant']}_16x128cubed_fp64.txt"), kernel)
+    pr.q = serial
+    pr.DoSerial()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.DoSerial()
+    torch.cuda.synchronize()
+    t_serial = time.perf_counter() - t0
+    pr.q = sol
+    pr.DoParallel()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = pr.DoParallel()
+    torch.cuda.synchronize()
+    t_par = time.perf_counter() - t0
+    K = rep.iterations
+    res = {"metric": METRIC, "value": round(lup / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (8 seeded Gaussians, SplitMix64 seed 0)",
+           "config": {"workload": "128^3 grid, 16 time slices batched, FP64 fine RK4 dt=5e-5 + FP64 coarse RK4 "
+                                  "dt=1/1008, K=5, T=1", "grid": N, "slices": S,
+                      "step": "one batched fused RK4 fine step over all 16 slices", "parallelism": "slices-as-batch"},
+           "roofline": rl,
+           "parareal": {"wall_s": round(t_par, 4), "iterations": K,
+                        "per_iteration": [{"fine_s": round(rep.t_fine[k], 4), "chain_s": round(rep.t_coarse[k], 4)}
+                                          for k in range(K)],
+                        "iteration_over_fine_sweep": round(sum(rep.t_fine[k] + rep.t_coarse[k] for k in range(K)) /
+                                                           sum(rep.t_fine[k] for k in range(K)), 4),
+                        "rel_l2_err": float(f"{sp.rel_l2(sol, serial):.3e}")},
+           "serial_fine_wall_s": round(t_serial, 4), "cpu_baseline": None}
+    print(json.dumps(res), flush=True)
+
+
+# ---------------------------------------------------------------------------------------
+# the uneven batch (configuration 5)
+
+def cfg5(a):
+    """24 independent problems (64^3 .. 160^3, 8 .. 24 slices, nu and c drawn from seed 0) in
+    one Parareal batch, F in FP64, G in FP32 (difference-form correction), tiles autotuned,
+    K = 5.  value = lattice updates of the solves (fine + coarse) per second; per-problem
+    error against its serial FP64 fine solution on the same GPU."""
     import torch
     import stencilparareal_amd as sp
     from stencilparareal_amd.config import uneven_batch
@@ -168,17 +389,14 @@ def uneven(a):
     ups = uneven_batch(24, seed=0)
     probs = [sp.Problem(u.n, u.slices, u.nu, u.c, u.end_time, u.steps_fine, u.steps_coarse) for u in ups]
     K = 5
-    log("uneven batch: create + autotune")
-    b = sp.Batch(probs, kmax=K, coarse_dtype="f32", use_graphs=bool(a.graphs), autotune=True)
+    b = sp.Batch(probs, kmax=K, coarse_dtype="f32", use_graphs=True, autotune=True)
     u0, q, ref = b.fields("u"), b.fields("q"), b.fields("serial")
     for f, u in zip(u0, ups):
         g = u.gauss
         sp.fill_gaussians(f, g.xc, g.yc, g.zc, g.sigma, g.amp)
-    log(f"warmup {a.warmup}")
     for _ in range(a.warmup):
         b.run(u0, q)
     torch.cuda.synchronize()
-    log(f"timed {a.steps}")
     t0 = time.perf_counter()
     for _ in range(a.steps):
         rep = b.run(u0, q)
@@ -192,14 +410,9 @@ def uneven(a):
         b.serial(i, u0[i], ref[i])
         torch.cuda.synchronize()
         t_serial += time.perf_counter() - t1
-        inf = b.info(i)
         per.append({"n": u.n, "slices": u.slices, "nu": float(f"{u.nu:.4e}"),
-                    "rel_l2_err": float(f"{sp.rel_l2(q[i], ref[i]):.3e}"),
-                    "tile": f"{ {1: 'pair', 2: 'quad'}.get(inf.fine_variant, 'scalar')} {inf.fine_tx}x{inf.fine_ty}x"
-                            f"{inf.fine_nthreads} k{inf.fine_kchunk}"})
-    # the batch stopped by its increment test at 1e-6 (every problem's increment below it)
-    log("uneven batch to tol 1e-6")
-    bt = sp.Batch(probs, kmax=K, coarse_dtype="f32", tol=1e-6, use_graphs=bool(a.graphs), autotune=True)
+                    "rel_l2_err": float(f"{sp.rel_l2(q[i], ref[i]):.3e}")})
+    bt = sp.Batch(probs, kmax=K, coarse_dtype="f32", tol=1e-6, use_graphs=True, autotune=True)
     qt = bt.fields("qt")
     bt.run(u0, qt)
     torch.cuda.synchronize()
@@ -208,41 +421,74 @@ def uneven(a):
     torch.cuda.synchronize()
     t_tol = time.perf_counter() - t1
     worst = max(sp.rel_l2(qt[i], ref[i]) for i in range(len(ups)))
-    to_tol = {"tol": 1e-6, "wall_s": round(t_tol, 4), "iterations": rep_t.iterations,
-              "max_rel_l2_err": float(f"{worst:.3e}"), "speedup_vs_serial_fine": round(t_serial / t_tol, 3)}
-    del bt
-    out = {
-        "metric": "Parareal wall-time vs serial-FP64-fine at \u22641e-6 rel L2; stencil GLUP/s + %HBM peak",
-        "value": round(rep.lattice_updates / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (4 seeded Gaussians per problem, SplitMix64 seed 0)",
-        "config": {"workload": "uneven batch: 24 independent problems, n in {64,96,128,160}, slices in {8,16,24}, "
-                               "nu log-uniform [1e-3,1e-2], c uniform [-1,1]^3, coarse RK4 dt = half its stability "
-                               "limit, fine dt = coarse/16, 2 coarse steps per slice, F FP64, G FP32, K=5",
-                   "problems": len(ups), "kmax": K, "parallelism": "problems-and-slices-as-batch"},
-        "iterations": rep.iterations, "max_increment_per_iter": [float(f"{x:.3e}") for x in rep.increment[:rep.iterations]],
-        "serial_fine_wall_s_all_problems": round(t_serial, 4), "batch_to_tol": to_tol, "per_problem": per,
-        "roofline": None, "cpu_baseline": None,
-    }
+    out = {"metric": METRIC, "value": round(rep.lattice_updates / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (4 seeded Gaussians per problem, SplitMix64 seed 0)",
+           "config": {"workload": "uneven batch: 24 independent problems, n in {64,96,128,160}, slices in "
+                                  "{8,16,24}, nu log-uniform [1e-3,1e-2], c uniform [-1,1]^3, coarse RK4 dt = half its "
+                                  "stability limit, fine dt = coarse/16, 2 coarse steps per slice, F FP64, G FP32, K=5",
+                      "problems": len(ups), "kmax": K, "parallelism": "problems-and-slices-as-batch",
+                      "step": "one batch solve"},
+           "iterations": rep.iterations,
+           "max_increment_per_iter": [float(f"{x:.3e}") for x in rep.increment[:rep.iterations]],
+           "serial_fine_wall_s_all_problems": round(t_serial, 4),
+           "batch_to_tol": {"tol": 1e-6, "wall_s": round(t_tol, 4), "iterations": rep_t.iterations,
+                            "max_rel_l2_err": float(f"{worst:.3e}"),
+                            "speedup_vs_serial_fine": round(t_serial / t_tol, 3)},
+           "per_problem": per, "roofline": None, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
+
+
+def roofline_only(a):
+    """Only the dominant kernel and a copy kernel of the same bytes (the PMC scale), for
+    profilers: a.reps batched steps of the workload's slices."""
+    import torch
+    import stencilparareal_amd as sp
+    from stencilparareal_amd.config import workload
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    w = workload(4 if a.workload == "cfg4" else 2)
+    N, S = w.N, w.slices
+    conf = conf_of(w)
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
+    if a.variant == "auto":
+        conv.autotune(S)
+    else:
+        # a fixed kernel: under counter collection the autotuner's timings are not the
+        # kernels' own, so profiles name the kernel they measure
+        conv.set_tile(32, 32, 512, 0, variant=a.variant)
+    init = conv.field("qinitial")
+    g = w.gauss
+    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    bt = Batch2(conv, init, S)
+    bt.run(2, conf.dtFine())
+    torch.cuda.synchronize()
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    bt.run(a.reps, conf.dtFine(), ev)
+    ev[1].synchronize()
+    k_ms = ev[0].elapsed_time(ev[1]) / a.reps
+    copy = bt.copy_gbs(reps=3)
+    alg = 16.0 * S * N ** 3
+    print(json.dumps({"kernel_avg_ms": round(k_ms, 4), "bytes_per_launch": alg, "tile": conv.get_tile(S, True),
+                      "copy_bytes_read": alg / 2, "copy_bytes_written": alg / 2, "copy_gbs": round(copy, 1),
+                      "reps": a.reps}), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-compare", action="store_true", help="skip comparator runs")
-    ap.add_argument("--graphs", type=int, default=1, help="capture each iteration as a hipGraph")
+    ap.add_argument("--no-solve", action="store_true", help="skip the once-timed Parareal, serial and comparator runs")
+    ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5"], default="cfg4",
+                    help="cfg4: 384^3 x 64 slices, BF16 G (default); cfg2: 128^3 x 16 slices, FP64; "
+                         "cfg5: the 24-problem uneven batch")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the batched RK4 step and a copy kernel of the same size (for profilers)")
-    ap.add_argument("--reps", type=int, default=20, help="launches per kernel with --roofline-only")
+    ap.add_argument("--reps", type=int, default=5, help="launches of the kernel with --roofline-only")
     ap.add_argument("--variant", choices=["auto", "quad", "pair"], default="auto",
                     help="fused kernel for --roofline-only: autotuned (default) or a fixed 32x32x512 variant")
-    ap.add_argument("--workload", choices=["parareal128", "uneven"], default="parareal128",
-                    help="parareal128: the 128^3 / 16-slice FP64 solve (default); uneven: the 24-problem batch")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the time slices are the batch "
@@ -251,196 +497,7 @@ def main():
         sys.exit(2)
     if a.roofline_only:
         return roofline_only(a)
-    if a.workload == "uneven":
-        return uneven(a)
-
-    import torch
-
-    import stencilparareal_amd as sp
-    from stencilparareal_amd import _lib as L
-    from stencilparareal_amd.config import RuntimeConfiguration, rounded_steps, workload
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    w = workload(2)
-    N, S = w.N, w.slices
-    conf = RuntimeConfiguration(gridSize=N, endTime=w.T, timeStepsFine=w.stepsF, timeStepsCoarse=w.stepsG,
-                                nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=w.kmax, async_=False,
-                                timeSlices=S)
-    tsF, tsG = conf.timeStepsFinePerTimeSlice(), conf.timeStepsCoarsePerTimeSlice()
-    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
-    init, sol, serial = conv.field("qinitial"), conv.field("q"), conv.field("qreference")
-    g = w.gauss
-    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
-    log("create (autotunes the fused tiles for these sizes)")
-    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype="f64", use_graphs=bool(a.graphs),
-                     autotune=True)
-
-    # serial FP64 fine solution on the GPU (also the error reference)
-    log("serial fine")
-    pr.q = serial
-    pr.DoSerial()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pr.DoSerial()
-    torch.cuda.synchronize()
-    t_serial = time.perf_counter() - t0
-    pr.q = sol
-
-    # the timed Parareal solves
-    log(f"warmup {a.warmup}")
-    for _ in range(a.warmup):
-        pr.DoParallel()
-    torch.cuda.synchronize()
-    log(f"timed {a.steps}")
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        rep = pr.DoParallel()
-    torch.cuda.synchronize()
-    t_steps = time.perf_counter() - t0
-    ms = t_steps / a.steps * 1e3
-
-    # updates the solve performs: K fine sweeps over S slices; coarse: the initial chain (S
-    # segments: S-1 slice starts, and the last slice's coarse value, which with constant nu
-    # also supplies every other G(U_r^0)) and S-1 segments per iteration (G(U_0) is reused)
-    lup_f = w.kmax * S * tsF * N ** 3
-    lup_g = tsG * N ** 3 * (S + w.kmax * (S - 1))
-    value = (lup_f + lup_g) / (ms / 1e3) / 1e9
-
-    # error of every iterate against the serial fine solution (untimed)
-    pr.begin()
-    errs_l2, errs_inf = [sp.rel_l2(pr.solution(), serial)], [sp.error_reference(pr.solution(), serial)]
-    for _ in range(w.kmax):
-        pr.iterate()
-        errs_l2.append(sp.rel_l2(pr.solution(), serial))
-        errs_inf.append(sp.error_reference(pr.solution(), serial))
-
-    # roofline of the dominant kernel, the batched fused RK4 step over all 16 slices: one
-    # more iteration of the same solve without a graph, its fine sweep (tsF launches back
-    # to back on the solve's own slice arrays) timed with HIP events on the launch stream
-    log("roofline")
-    conf1 = RuntimeConfiguration(gridSize=N, endTime=w.T, timeStepsFine=w.stepsF, timeStepsCoarse=w.stepsG,
-                                 nu0=w.nu, cx=w.c[0], cy=w.c[1], cz=w.c[2], kmax=1, async_=False, timeSlices=S)
-    sol1 = conv.field("q1")
-    pr1 = sp.Parareal(conv, init, sol1, conf1, coarse="rk4", coarse_dtype="f64", use_graphs=False)
-    rep1 = pr1.DoParallel()
-    k_ms = rep1.ms_fine_events[0] / tsF
-    alg_bytes = 16.0 * S * N ** 3  # FP64 state read once + written once per lattice update
-    del pr1, sol1
-    achieved = alg_bytes / (k_ms / 1e3) / 1e9
-    tile = conv.get_tile(S, True)
-    traffic = committed_traffic(tile["variant"])
-
-    # the metric's wall time: the same solve stopped by its own increment test at 1e-6
-    # (untimed in `value`, which is the configured K = 5 workload)
-    log("parareal to tol 1e-6")
-    sol_t = conv.field("q_tol")
-    pr_t = sp.Parareal(conv, init, sol_t, conf, coarse="rk4", coarse_dtype="f64", use_graphs=bool(a.graphs),
-                       tol=1e-6)
-    pr_t.DoParallel()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rep_t = pr_t.DoParallel()
-    torch.cuda.synchronize()
-    t_tol = time.perf_counter() - t0
-    to_tol = {"tol": 1e-6, "wall_s": round(t_tol, 4), "iterations": rep_t.iterations,
-              "rel_l2_err": float(f"{sp.rel_l2(sol_t, serial):.3e}"),
-              "speedup_vs_serial_fine": round(t_serial / t_tol, 3)}
-    del pr_t, sol_t
-
-    extra = {}
-    if not a.no_compare:
-        log("mixed-precision coarse comparator")
-        sol2 = conv.field("q2")
-        pr32 = sp.Parareal(conv, init, sol2, conf, coarse="rk4", coarse_dtype="f32", use_graphs=bool(a.graphs))
-        pr32.DoParallel()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pr32.DoParallel()
-        torch.cuda.synchronize()
-        extra["parareal_f32_coarse_s"] = round(time.perf_counter() - t0, 4)
-        extra["parareal_f32_coarse_rel_l2_at_K"] = sp.rel_l2(sol2, serial)
-        del pr32
-
-        # The configuration's own target: batched Parareal with FP32 G against plain all-FP64
-        # Parareal on the same GPU, both run until the 1e-6 increment test stops them.  The
-        # plain one is the reference's execution structure: reference-order sweeps (4 RHS + 3
-        # Euler + 1 RK kernel per step), slice after slice, FP64 G.
-        def to_tol_run(c, gdt, field_name, graphs):
-            sol = c.field(field_name)
-            pr_x = sp.Parareal(c, init, sol, conf, coarse="rk4", coarse_dtype=gdt, use_graphs=graphs, tol=1e-6)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            rep_x = pr_x.DoParallel()
-            torch.cuda.synchronize()
-            out_x = {"wall_s": round(time.perf_counter() - t1, 4), "iterations": rep_x.iterations,
-                     "rel_l2_err": float(f"{sp.rel_l2(sol, serial):.3e}")}
-            del pr_x
-            return out_x
-        log("batched FP32-G Parareal to tol 1e-6")
-        to_tol_run(conv, "f32", "q_f32t", bool(a.graphs))  # warm-up (graph capture, first launches)
-        mixed = to_tol_run(conv, "f32", "q_f32t", bool(a.graphs))
-        log("plain all-FP64 Parareal to tol 1e-6 (reference-order sweeps, slice by slice)")
-        conv_plain = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz, path="staged")
-        plain = to_tol_run(conv_plain, "f64", "q_plain", False)  # host-launched, as the reference
-        del conv_plain
-        extra["target_to_tol_1e-6"] = {
-            "batched_f32_coarse": mixed, "plain_fp64_reference_structure": plain,
-            "speedup": round(plain["wall_s"] / mixed["wall_s"], 3),
-            "speedup_batched_f64_coarse": round(plain["wall_s"] / t_tol, 3)}
-
-    cpu = None
-    if not a.no_cpu:
-        log("cpu baseline")
-        cpu = cpu_baseline(w)
-
-    out = {
-        "metric": "Parareal wall-time vs serial-FP64-fine at ≤1e-6 rel L2; stencil GLUP/s + %HBM peak",
-        "value": round(value, 3),
-        "unit": "GLUP/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(ms, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (8 seeded Gaussians, SplitMix64 seed 0)",
-        "config": {"workload": "128^3 grid, 16 time slices batched, FP64 fine RK4 dt=5e-5 + FP64 coarse RK4 "
-                               "dt=1/1008, K=5, T=1", "grid": N, "slices": S, "kmax": w.kmax,
-                   "fine_steps_per_slice": tsF, "coarse_steps_per_slice": tsG, "parallelism": "slices-as-batch"},
-        "parareal_wall_s": round(ms / 1e3, 4),
-        "serial_fine_wall_s": round(t_serial, 4),
-        "speedup_vs_serial_fine": round(t_serial / (ms / 1e3), 4),
-        "parareal_to_tol": to_tol,
-        "rel_l2_err_per_iterate": [float(f"{e:.4e}") for e in errs_l2],
-        "rel_linf_err_per_iterate": [float(f"{e:.4e}") for e in errs_inf],
-        "fine_glups_serial": round(N ** 3 * conf.timeStepsFine / t_serial / 1e9, 3),
-        # with graphs an iteration (batched F + G chain + correction) is one launch, so its
-        # split is the batched step's kernel time x fine steps per slice vs the remainder
-        "phase_s": {"init_coarse": round(rep.t_init, 4),
-                    "iteration": [round(f + c, 4) for f, c in zip(rep.t_fine[:rep.iterations],
-                                                                   rep.t_coarse[:rep.iterations])],
-                    "fine_batched_kernel_est": round(k_ms * tsF / 1e3, 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_rk4_fused_{tile['variant']} {tile['tx']}x{tile['ty']} tiles, "
-                               f"{tile['nthreads']} threads (FP64, {S} slices of {N}^3, kchunk {tile['kchunk']})",
-                     "avg_launch_ms": round(k_ms, 4), "bytes_per_launch": alg_bytes},
-        "cpu_baseline": cpu,
-    }
-    out.update(extra)
-    print(json.dumps(out), flush=True)
-
-
-def C_ptr(lay):
-    import ctypes
-    return ctypes.byref(lay)
-
-
-def ctypes_vp(t):
-    import ctypes
-    return ctypes.c_void_p(t.data_ptr())
+    return {"cfg4": cfg4, "cfg2": cfg2, "cfg5": cfg5}[a.workload](a)
 
 
 if __name__ == "__main__":

@@ -85,8 +85,10 @@ public:
         check(sp_parareal_run(pr_, &initial_.layout(), initial_.device(), &q_.layout(), q_.device(), &report_));
     }
 
-    // 5*kmax+2 cumulative stamps as in Parareal.h:173-248 (receive and send take no time
-    // on one device; the coarse stamp includes the update).
+    // 5*kmax+2 cumulative stamps as in Parareal.h:173-248: after the fine sweep, the
+    // receive, the coarse propagation, the update and the send.  Fine and chain times are HIP
+    // events on the stream; the chain splits into coarse and update where the corrections
+    // were timed (without graphs).  Receive and send take no time on one device.
     void DoTimedParallel(std::vector<double>& times)
     {
         DoParallel();
@@ -96,10 +98,11 @@ public:
         for (int k = 0; k < kmax_; ++k) {
             if (k < report_.iterations) {
                 t += report_.t_fine[k];
-                times[5 * k + 1] = t;
-                times[5 * k + 2] = t;
-                t += report_.t_coarse[k];
-                times[5 * k + 3] = times[5 * k + 4] = times[5 * k + 5] = t;
+                times[5 * k + 1] = times[5 * k + 2] = t;
+                t += report_.t_coarse[k] - report_.t_update[k];
+                times[5 * k + 3] = t;
+                t += report_.t_update[k];
+                times[5 * k + 4] = times[5 * k + 5] = t;
             } else {
                 for (int q = 1; q <= 5; ++q) times[5 * k + q] = t;
             }

@@ -196,7 +196,8 @@ typedef struct sp_parareal_cfg {
     int coarse_substeps; /* >= 1: each coarse step is this many sub-steps */
     double tol;          /* stop once the relative L2 increment <= tol; 0: run kmax */
     int async_schedule;  /* 1: fine-step times of the reference's async mode */
-    int use_graphs;      /* 1: capture each iteration (fine sweep, coarse chain, updates) as one hipGraph */
+    int use_graphs;      /* 1: capture each iteration as two hipGraphs, the fine sweep and the coarse
+                            chain with its updates, replayed back to back */
     int autotune;        /* 1: autotune the fused-kernel tiles of F and G at create (sp_prop_autotune) */
     int delta_correction; /* 0: the reference's update G(U_n^{k+1}) + F(U_n^k) - G(U_n^k)
                              (Parareal.h:19-35); 1: F(U_n^k) + G(U_n^{k+1} - U_n^k), equal for
@@ -210,12 +211,15 @@ typedef struct sp_parareal_report {
     int steps_fine_per_slice, steps_coarse_per_slice;
     double dt_fine, dt_coarse;         /* effective steps after rounding */
     double increment[SP_MAX_ITERS];    /* relative L2 change of the slice states */
-    double t_init;                     /* seconds: initial coarse sweep */
-    double t_fine[SP_MAX_ITERS];       /* seconds per iteration: batched fine sweep */
-    double t_coarse[SP_MAX_ITERS];     /* coarse chain incl. updates */
-    double t_total;
-    double ms_fine_events[SP_MAX_ITERS]; /* ms: the fine sweep between HIP events on the
-                                            stream (iterations run without a graph; 0 with) */
+    double t_init;                     /* seconds: initial coarse sweep (wall clock) */
+    double t_fine[SP_MAX_ITERS];       /* seconds per iteration: the batched fine sweep, between
+                                          HIP events on the stream (with use_graphs the sweep and
+                                          the chain are two graphs, timed the same way) */
+    double t_coarse[SP_MAX_ITERS];     /* seconds: the coarse chain incl. its corrections (events) */
+    double t_total;                    /* seconds: the whole run (wall clock) */
+    double ms_fine_events[SP_MAX_ITERS]; /* ms: t_fine in milliseconds */
+    double t_update[SP_MAX_ITERS];     /* seconds: the corrections alone, each launch between
+                                          events (iterations without a graph; 0 with graphs) */
 } sp_parareal_report;
 
 int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out);
@@ -226,6 +230,11 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0);
 /* One iteration (Parareal.h:116-165).  *increment (nullable) receives the
  * relative L2 change of all slice states.  Synchronous. */
 int sp_parareal_iterate(sp_parareal* pr, double* increment);
+/* Phase times of the last sp_parareal_iterate, in seconds, between HIP events on the
+ * stream: the fine sweep, the coarse chain with its corrections, and the corrections alone
+ * (0 when the iteration ran as graphs).  Outputs may be NULL.  Counterpart of the per-
+ * iteration stamps of Parareal::DoTimedParallel (Parareal.h:173-248). */
+int sp_parareal_iteration_times(const sp_parareal* pr, double* fine_s, double* chain_s, double* update_s);
 /* Copies the current last-slice solution (the last rank's q) into q. */
 int sp_parareal_solution(sp_parareal* pr, const sp_layout* lq, void* q);
 /* DoParallel: begin + up to kmax iterations (stopping at tol) + solution. */

@@ -36,7 +36,8 @@ EXPORTS = [
     "sp_do_euler", "sp_do_rk4_timestep", "sp_do_euler_timestep", "sp_do_rk4_ex", "sp_do_rk4_batched",
     "sp_periodic_fill",
     "sp_parareal_update", "sp_norms", "sp_fill_exact", "sp_fill_gaussians", "sp_parareal_create",
-    "sp_parareal_destroy", "sp_parareal_begin", "sp_parareal_iterate", "sp_parareal_solution",
+    "sp_parareal_destroy", "sp_parareal_begin", "sp_parareal_iterate", "sp_parareal_iteration_times",
+    "sp_parareal_solution",
     "sp_parareal_run", "sp_parareal_serial",
     "sp_batch_create", "sp_batch_destroy", "sp_batch_run", "sp_batch_increments", "sp_batch_problem_info",
     "sp_batch_serial",
@@ -69,7 +70,7 @@ class PararealReport(C.Structure):
                 ("dt_coarse", C.c_double), ("increment", C.c_double * SP_MAX_ITERS),
                 ("t_init", C.c_double), ("t_fine", C.c_double * SP_MAX_ITERS),
                 ("t_coarse", C.c_double * SP_MAX_ITERS), ("t_total", C.c_double),
-                ("ms_fine_events", C.c_double * SP_MAX_ITERS)]
+                ("ms_fine_events", C.c_double * SP_MAX_ITERS), ("t_update", C.c_double * SP_MAX_ITERS)]
 
 
 class Problem(C.Structure):
@@ -136,6 +137,7 @@ _SIGS = {
     "sp_parareal_destroy": (C.c_int, [_vp]),
     "sp_parareal_begin": (C.c_int, [_vp, _lp, _vp]),
     "sp_parareal_iterate": (C.c_int, [_vp, _dp]),
+    "sp_parareal_iteration_times": (C.c_int, [_vp, _dp, _dp, _dp]),
     "sp_parareal_solution": (C.c_int, [_vp, _lp, _vp]),
     "sp_parareal_run": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.POINTER(PararealReport)]),
     "sp_parareal_serial": (C.c_int, [_vp, _lp, _vp, _lp, _vp]),

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <utility>
@@ -45,6 +46,7 @@ int check_layout(const sp_layout* l, const char* what)
 }  // namespace sp
 
 using namespace sp;
+
 
 namespace {
 
@@ -725,11 +727,15 @@ struct sp_parareal {
     int iters_done = 0;
     double u0_norm2 = 0.;  // sum of u0^2, for the blow-up check
     FusedTile tileF{}, tileG1{}, tileGS{};
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    // With use_graphs an iteration is two graphs, the fine sweep and the coarse chain (its
+    // coarse runs and corrections), launched back to back with events between them.
+    hipGraphExec_t gexecF = nullptr, gexecC = nullptr;
+    hipGraph_t graphF = nullptr, graphC = nullptr;
     std::vector<double> part_host;
-    hipEvent_t ev[2] = {nullptr, nullptr};  // around the fine sweep (iterations without a graph)
-    double fine_ms = 0.;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // before the sweep, after it, after the chain
+    // without graphs: events around every correction launch of the chain (2 per slice)
+    std::vector<hipEvent_t> evU;
+    double fine_ms = 0., chain_ms = 0., update_ms = 0.;
 };
 
 namespace {
@@ -831,7 +837,14 @@ int fine_sweep(sp_parareal* pr)
     return SP_OK;
 }
 
-int coarse_chain(sp_parareal* pr)
+// Events around the correction of slice r (timed = true only outside graph capture).
+int mark_update(sp_parareal* pr, int r, int which, bool timed)
+{
+    if (timed) SP_HIP(hipEventRecord(pr->evU[2 * r + which], pr->p->ctx->stream));
+    return SP_OK;
+}
+
+int coarse_chain(sp_parareal* pr, bool timed)
 {
     sp_prop* p = pr->p;
     hipStream_t s = p->ctx->stream;
@@ -852,8 +865,10 @@ int coarse_chain(sp_parareal* pr)
         const View fine = cview((char*)pr->F.p + (size_t)r * pr->n * 8, p->ni, p->nj);
         const View dst = (r < pr->S - 1) ? cview((char*)pr->U.p + (size_t)(r + 1) * pr->n * 8, p->ni, p->nj)
                                          : cview(pr->Q.p, p->ni, p->nj);
+        SP_TRY(mark_update(pr, r, 0, timed));
         SP_TRY(launch_correction(s, p->ni, p->nj, p->nk, pr->gdt, cnew, cold, fine, dst,
                                  (double*)pr->part.p + (size_t)r * kReduceBlocks * 2, kReduceBlocks));
+        SP_TRY(mark_update(pr, r, 1, timed));
     }
     return SP_OK;
 }
@@ -861,7 +876,7 @@ int coarse_chain(sp_parareal* pr)
 // The chain in difference form (delta_correction): U_{r+1} = F(U_r^k) + G(U_r^{k+1} - U_r^k),
 // equal to the reference's G(U_r^{k+1}) + F(U_r^k) - G(U_r^k) for the linear G.  D carries
 // the difference from one slice to the next; slice 0's is zero (its start state is u0).
-int coarse_chain_delta(sp_parareal* pr)
+int coarse_chain_delta(sp_parareal* pr, bool timed)
 {
     sp_prop* p = pr->p;
     hipStream_t s = p->ctx->stream;
@@ -877,9 +892,11 @@ int coarse_chain_delta(sp_parareal* pr)
         const bool last = r == pr->S - 1;
         const View dst = last ? cview(pr->Q.p, p->ni, p->nj)
                               : cview((char*)pr->U.p + (size_t)(r + 1) * pr->n * 8, p->ni, p->nj);
+        SP_TRY(mark_update(pr, r, 0, timed));
         SP_TRY(launch_correction_delta(s, p->ni, p->nj, p->nk, pr->gdt, gd, fine, dst,
                                        last ? cview(nullptr, p->ni, p->nj) : D,
                                        (double*)pr->part.p + (size_t)r * kReduceBlocks * 2, kReduceBlocks));
+        SP_TRY(mark_update(pr, r, 1, timed));
     }
     return SP_OK;
 }
@@ -1017,9 +1034,13 @@ int sp_parareal_destroy(sp_parareal* pr)
 {
     if (!pr) return SP_OK;
     (void)hipStreamSynchronize(pr->p->ctx->stream);
-    if (pr->gexec) (void)hipGraphExecDestroy(pr->gexec);
-    if (pr->graph) (void)hipGraphDestroy(pr->graph);
+    for (hipGraphExec_t g : {pr->gexecF, pr->gexecC})
+        if (g) (void)hipGraphExecDestroy(g);
+    for (hipGraph_t g : {pr->graphF, pr->graphC})
+        if (g) (void)hipGraphDestroy(g);
     for (hipEvent_t e : pr->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : pr->evU)
         if (e) (void)hipEventDestroy(e);
     for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->coefF, &pr->coefG, &pr->part,
                    &pr->D})
@@ -1117,59 +1138,82 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
     return SP_OK;
 }
 
+// Captures `body` (enqueued on stream s) as a graph and instantiates it.
+static int capture(sp_parareal* pr, hipStream_t s, hipGraph_t* graph, hipGraphExec_t* exec,
+                   const std::function<int()>& body)
+{
+    sp_prop* p = pr->p;
+    const hipStream_t user = p->ctx->stream;
+    // begin the capture first: on failure the context still follows the caller's stream
+    SP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    p->ctx->stream = s;
+    const int r = body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    p->ctx->stream = user;
+    if (r != SP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    SP_HIP(e);
+    *graph = g;
+    SP_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return SP_OK;
+}
+
 static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tcoarse)
 {
     sp_prop* p = pr->p;
     hipStream_t s = p->ctx->stream;
-    const double a = now();
-    if (pr->cfg.use_graphs) {
-        // Capture needs a created stream: the null stream is first drained, then the graph
-        // runs on the context's own stream.
-        hipStream_t user = s;
-        if (user == nullptr) {
+    for (hipEvent_t& e : pr->ev)
+        if (!e) SP_HIP(hipEventCreate(&e));
+    const bool graphs = pr->cfg.use_graphs != 0;
+    if (graphs) {
+        // Capture needs a created stream: the null stream is first drained, then the graphs
+        // run on the context's own stream.
+        if (s == nullptr) {
             SP_HIP(hipStreamSynchronize(nullptr));
             s = p->ctx->own;
         }
-        if (!pr->gexec) {
-            // begin the capture first: on failure the context still follows the caller's stream
-            SP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            p->ctx->stream = s;
-            int r = fine_sweep(pr);
-            if (r == SP_OK) r = (pr->cfg.delta_correction ? coarse_chain_delta(pr) : coarse_chain(pr));
-            hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(s, &g);
-            p->ctx->stream = user;
-            if (r != SP_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return r;
-            }
-            SP_HIP(e);
-            pr->graph = g;
-            SP_HIP(hipGraphInstantiate(&pr->gexec, g, nullptr, nullptr, 0));
+        if (!pr->gexecF) {
+            SP_TRY(capture(pr, s, &pr->graphF, &pr->gexecF, [&] { return fine_sweep(pr); }));
+            SP_TRY(capture(pr, s, &pr->graphC, &pr->gexecC, [&] {
+                return pr->cfg.delta_correction ? coarse_chain_delta(pr, false) : coarse_chain(pr, false);
+            }));
         }
-        SP_HIP(hipGraphLaunch(pr->gexec, s));
-        SP_HIP(hipStreamSynchronize(s));
-        if (tfine) *tfine = now() - a;
-        if (tcoarse) *tcoarse = 0.;
-        pr->fine_ms = 0.;
+        SP_HIP(hipEventRecord(pr->ev[0], s));
+        SP_HIP(hipGraphLaunch(pr->gexecF, s));
+        SP_HIP(hipEventRecord(pr->ev[1], s));
+        SP_HIP(hipGraphLaunch(pr->gexecC, s));
+        SP_HIP(hipEventRecord(pr->ev[2], s));
     } else {
-        if (!pr->ev[0]) {
-            SP_HIP(hipEventCreate(&pr->ev[0]));
-            SP_HIP(hipEventCreate(&pr->ev[1]));
+        if (pr->evU.empty()) {
+            pr->evU.assign((size_t)2 * pr->S, nullptr);
+            for (hipEvent_t& e : pr->evU) SP_HIP(hipEventCreate(&e));
         }
         SP_HIP(hipEventRecord(pr->ev[0], s));
         SP_TRY(fine_sweep(pr));
         SP_HIP(hipEventRecord(pr->ev[1], s));
-        SP_HIP(hipStreamSynchronize(s));
-        float ms = 0.f;
-        SP_HIP(hipEventElapsedTime(&ms, pr->ev[0], pr->ev[1]));
-        pr->fine_ms = ms;
-        const double b = now();
-        SP_TRY((pr->cfg.delta_correction ? coarse_chain_delta(pr) : coarse_chain(pr)));
-        SP_HIP(hipStreamSynchronize(s));
-        if (tfine) *tfine = b - a;
-        if (tcoarse) *tcoarse = now() - b;
+        SP_TRY((pr->cfg.delta_correction ? coarse_chain_delta(pr, true) : coarse_chain(pr, true)));
+        SP_HIP(hipEventRecord(pr->ev[2], s));
     }
+    SP_HIP(hipStreamSynchronize(s));
+    // phase times from the events on the stream: the fine sweep, and the coarse chain with
+    // its corrections (the corrections alone when each was bracketed)
+    float f_ms = 0.f, c_ms = 0.f;
+    SP_HIP(hipEventElapsedTime(&f_ms, pr->ev[0], pr->ev[1]));
+    SP_HIP(hipEventElapsedTime(&c_ms, pr->ev[1], pr->ev[2]));
+    pr->fine_ms = f_ms;
+    pr->chain_ms = c_ms;
+    pr->update_ms = 0.;
+    if (!graphs)
+        for (int r = 0; r < pr->S; ++r) {
+            float u = 0.f;
+            SP_HIP(hipEventElapsedTime(&u, pr->evU[2 * r], pr->evU[2 * r + 1]));
+            pr->update_ms += u;
+        }
+    if (tfine) *tfine = f_ms * 1e-3;
+    if (tcoarse) *tcoarse = c_ms * 1e-3;
     SP_HIP(hipMemcpy(pr->part_host.data(), pr->part.p, pr->part_host.size() * sizeof(double), hipMemcpyDeviceToHost));
     double d2 = 0., q2 = 0.;
     for (size_t i = 0; i < pr->part_host.size(); i += 2) {
@@ -1195,6 +1239,16 @@ int sp_parareal_iterate(sp_parareal* pr, double* increment)
     SP_CHECK(pr->begun, SP_ERR_INVALID, "sp_parareal_iterate: call sp_parareal_begin first");
     SP_HIP(hipSetDevice(pr->p->ctx->device));
     return iterate_impl(pr, increment, nullptr, nullptr);
+}
+
+int sp_parareal_iteration_times(const sp_parareal* pr, double* fine_s, double* chain_s, double* update_s)
+{
+    SP_CHECK(pr, SP_ERR_INVALID, "sp_parareal_iteration_times: null handle");
+    SP_CHECK(pr->iters_done > 0, SP_ERR_INVALID, "sp_parareal_iteration_times: no iteration has run");
+    if (fine_s) *fine_s = pr->fine_ms * 1e-3;
+    if (chain_s) *chain_s = pr->chain_ms * 1e-3;
+    if (update_s) *update_s = pr->update_ms * 1e-3;
+    return SP_OK;
 }
 
 int sp_parareal_solution(sp_parareal* pr, const sp_layout* lq, void* q)
@@ -1233,6 +1287,7 @@ int sp_parareal_run(sp_parareal* pr, const sp_layout* l0, const void* u0, const 
             rep->t_fine[k] = tf;
             rep->t_coarse[k] = tc;
             rep->ms_fine_events[k] = pr->fine_ms;
+            rep->t_update[k] = pr->update_ms * 1e-3;
             rep->iterations = k + 1;
         }
         if (pr->cfg.tol > 0 && inc <= pr->cfg.tol) {

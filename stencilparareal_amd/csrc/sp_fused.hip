@@ -616,7 +616,7 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         PM4::at(tid + m * NT, ax, by);
         o4[m] = (pmod(x0 + ax, ni) + pmod(y0 + by, nj) * osj) * 8;  // periodic image for partial tiles
     }
-    int s2[M2], w3[M2], r3[M3], w4[M3], r4[M4], n4[M4];
+    int s2[M2], w3[M2], r3x[M3], w4[M3], r4x[M4], n4[M4];
     // slots past a stage's last pair and the padding slots of padded rows repeat a pair:
     // they write nothing (several lanes writing one LDS address serialise).  The 32x32
     // tile has no registers left for the masks and writes the repeats.
@@ -634,17 +634,19 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
     }
     const auto wr0 = [&](int m) { return !MASK || m < M0 - 1 || tid + m * NT < A0P; };  // loader tail
     pair_qidx<TX, TY, 2, NT, M2>(s2, w3);
-    pair_qidx<TX, TY, 3, NT, M3>(r3, w4);
-    pair_qidx<TX, TY, 4, NT, M4>(r4, n4);
+    pair_qidx<TX, TY, 3, NT, M3>(r3x, w4);
+    pair_qidx<TX, TY, 4, NT, M4>(r4x, n4);
 
     double acc1[M1][2][4], acc2[M2][2][4], acc3[M3][2][4], acc4[M4][2][4];
     double d1[M1][2], d2v[M2][2], d3[M3][2], d4[M4][2];
     d2 qv2[M2];
     d2 c1[M1], c2[M2], c3[M3], c4[M4];                   // centres read this plane
-    d2 r1a[M1], r1b[M1], r3a[M3], r3b[M3], r4a[M4], r4b[M4];  // q one and two planes back
+    // q one and two planes back, in two-slot rings indexed by the plane's parity (R & 1 in
+    // the unrolled loop): slot B holds the value stored two planes ago
+    d2 r1[2][M1], r3[2][M3], r4[2][M4];
 #pragma unroll
     for (int m = 0; m < M1; ++m) {
-        r1a[m] = r1b[m] = d2{0., 0.};
+        r1[0][m] = r1[1][m] = d2{0., 0.};
         for (int e = 0; e < 2; ++e) acc1[m][e][0] = acc1[m][e][1] = acc1[m][e][2] = acc1[m][e][3] = 0.;
     }
 #pragma unroll
@@ -652,12 +654,12 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         for (int e = 0; e < 2; ++e) acc2[m][e][0] = acc2[m][e][1] = acc2[m][e][2] = acc2[m][e][3] = 0.;
 #pragma unroll
     for (int m = 0; m < M3; ++m) {
-        r3a[m] = r3b[m] = d2{0., 0.};
+        r3[0][m] = r3[1][m] = d2{0., 0.};
         for (int e = 0; e < 2; ++e) acc3[m][e][0] = acc3[m][e][1] = acc3[m][e][2] = acc3[m][e][3] = 0.;
     }
 #pragma unroll
     for (int m = 0; m < M4; ++m) {
-        r4a[m] = r4b[m] = d2{0., 0.};
+        r4[0][m] = r4[1][m] = d2{0., 0.};
         for (int e = 0; e < 2; ++e) acc4[m][e][0] = acc4[m][e][1] = acc4[m][e][2] = acc4[m][e][3] = 0.;
     }
 
@@ -684,6 +686,7 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 
     auto plane = [&](auto Rc, int t) {
         constexpr int R = decltype(Rc)::value;
+        constexpr int B = R & 1;  // ring slot of this plane's parity
         const int cur = t & 1, prv = cur ^ 1;
         __builtin_amdgcn_sched_barrier(0);  // keep planes apart: bounded live ranges
         if (t + 1 < tq1) {
@@ -698,15 +701,14 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         }
         __builtin_amdgcn_sched_barrier(0);
         if (t < tq1) {
-            // centres of plane t are q(t); r1b = q(t-2) is the update's q
+            // centres of plane t are q(t); r1[B] = q(t-2) is the update's q
             pair_step<TX, TY, 1, NT, M1, R & 3>(P0[cur], w, acc1, d1, c1);
 #pragma unroll
             for (int m = 0; m < M1; ++m) {
                 if (!MASK || lv1[m])
                     *(d2*)&P1[cur][PM1::out(tid + m * NT)] =
-                        d2{fmaT(w.f1, d1[m][0], r1b[m].x), fmaT(w.f1, d1[m][1], r1b[m].y)};
-                r1b[m] = r1a[m];
-                r1a[m] = c1[m];
+                        d2{fmaT(w.f1, d1[m][0], r1[B][m].x), fmaT(w.f1, d1[m][1], r1[B][m].y)};
+                r1[B][m] = c1[m];
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -727,14 +729,13 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
             pair_step<TX, TY, 3, NT, M3, (R + 2) & 3>(P2[prv], w, acc3, d3, c3);
 #pragma unroll
             for (int m = 0; m < M3; ++m) {
-                const d2 qn = *(const d2*)&Q3[prv][r3[m]];  // q(t-6), written by stage 2 last plane
+                const d2 qn = *(const d2*)&Q3[prv][r3x[m]];  // q(t-6), written by stage 2 last plane
                 if (!MASK || lv3[m]) {
                     *(d2*)&P3[cur][PM3::out(tid + m * NT)] =
-                        d2{fmaT(w.f3, d3[m][0], r3b[m].x), fmaT(w.f3, d3[m][1], r3b[m].y)};
-                    *(d2*)&Q4[cur][w4[m]] = r3b[m];  // q(t-8) for stage 4
+                        d2{fmaT(w.f3, d3[m][0], r3[B][m].x), fmaT(w.f3, d3[m][1], r3[B][m].y)};
+                    *(d2*)&Q4[cur][w4[m]] = r3[B][m];  // q(t-8) for stage 4
                 }
-                r3b[m] = r3a[m];
-                r3a[m] = qn;
+                r3[B][m] = qn;
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -746,13 +747,10 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 #pragma unroll
                 for (int m = 0; m < M4; ++m)
                     if (!MASK || lv4[m])
-                        st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], r4b[m].x), fmaT(w.f4, d4[m][1], r4b[m].y)});
+                        st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], r4[B][m].x), fmaT(w.f4, d4[m][1], r4[B][m].y)});
             }
 #pragma unroll
-            for (int m = 0; m < M4; ++m) {
-                r4b[m] = r4a[m];
-                r4a[m] = *(const d2*)&Q4[prv][r4[m]];  // q(t-9)
-            }
+            for (int m = 0; m < M4; ++m) r4[B][m] = *(const d2*)&Q4[prv][r4x[m]];  // q(t-9)
             __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();

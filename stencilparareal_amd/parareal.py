@@ -70,9 +70,11 @@ class Parareal:
         return rep
 
     def DoTimedParallel(self, times: list) -> L.PararealReport:
-        """Parareal.h:173-248: 5*kmax+2 cumulative stamps (init, then per iteration
-        after fine, receive, coarse, update, send; receive and send take no time on one
-        device, and the coarse stamp includes the update)."""
+        """Parareal.h:173-248: 5*kmax+2 cumulative stamps (init, then per iteration after
+        fine, receive, coarse, update, send).  Fine and coarse-chain times come from HIP
+        events on the stream; the chain splits into coarse propagation and update where the
+        corrections were timed (iterations without graphs).  Receive and send move no data
+        on one device and take no time."""
         rep = self.DoParallel()
         times.clear()
         t = rep.t_init
@@ -80,10 +82,11 @@ class Parareal:
         for k in range(self.conf.kmax):
             if k < rep.iterations:
                 t += rep.t_fine[k]
+                times.extend([t, t])
+                t += rep.t_coarse[k] - rep.t_update[k]
                 times.append(t)
-                times.append(t)
-                t += rep.t_coarse[k]
-                times.extend([t, t, t])
+                t += rep.t_update[k]
+                times.extend([t, t])
             else:
                 times.extend([t] * 5)
         times.append(rep.t_total)
@@ -104,6 +107,13 @@ class Parareal:
         inc = C.c_double()
         L.call("sp_parareal_iterate", self.h, C.byref(inc))
         return inc.value
+
+    def iteration_times(self) -> tuple[float, float, float]:
+        """(fine sweep, coarse chain with corrections, corrections alone) of the last
+        iterate() in seconds, from HIP events; the last is 0 for graph iterations."""
+        f, c, u = C.c_double(), C.c_double(), C.c_double()
+        L.call("sp_parareal_iteration_times", self.h, C.byref(f), C.byref(c), C.byref(u))
+        return f.value, c.value, u.value
 
     def solution(self, out: ConvectionField | None = None) -> ConvectionField:
         out = out or self.q

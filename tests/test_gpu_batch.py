@@ -146,3 +146,61 @@ def test_batch_mixed_kernels_match_oracle(sp):
         assert e <= 1e-12, (i, e)
     assert variants == {1, 2}, variants  # grouped pair and 2x2 kernels both ran
 
+
+
+def _uneven_fields(sp, b, ups):
+    u0, q = b.fields("u"), b.fields("q")
+    for f, u in zip(u0, ups):
+        g = u.gauss
+        sp.fill_gaussians(f, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    return u0, q
+
+
+def test_cfg5_uneven_batch_as_stated(sp):
+    # the uneven-batch configuration exactly as the benchmark runs it: 24 problems drawn
+    # with seed 0 (n in {64, 96, 128, 160}, S in {8, 16, 24}, nu log-uniform in [1e-3, 1e-2],
+    # c uniform in [-1, 1]^3), F in FP64, G in FP32 (difference-form correction), tiles
+    # autotuned, one hipGraph per iteration, K = 5
+    ups = uneven_batch(24, seed=0)
+    assert len(ups) == 24 and {u.n for u in ups} <= {64, 96, 128, 160} and {u.slices for u in ups} <= {8, 16, 24}
+    probs = [sp.Problem(u.n, u.slices, u.nu, u.c, u.end_time, u.steps_fine, u.steps_coarse) for u in ups]
+    K = 5
+    b = sp.Batch(probs, kmax=K, coarse_dtype="f32", use_graphs=True, autotune=True)
+    u0, q = _uneven_fields(sp, b, ups)
+    rep = b.run(u0, q)
+    assert rep.iterations == K
+    inc = list(rep.increment[:K])
+    print("largest increment per iteration", ["%.2e" % x for x in inc])
+    # the largest increment over the problems falls every iteration until the FP64 floor
+    assert all(y < x or y <= 1e-13 for x, y in zip(inc, inc[1:])), inc
+    ref = b.fields("serial")
+    worst = 0.0
+    for i, u in enumerate(ups):
+        b.serial(i, u0[i], ref[i])
+        e = sp.rel_l2(q[i], ref[i])
+        worst = max(worst, e)
+        assert e <= 1e-10, (i, u.n, u.slices, e)
+    print(f"24 problems: worst rel L2 against the serial FP64 fine solutions {worst:.2e}")
+
+
+def test_cfg5_uneven_batch_fp64_coarse_vs_oracle(sp):
+    # the same 24 problems (grids, slice counts, nu, c, initial fields) with FP64 G on a
+    # shortened horizon: 2 fine steps of the configuration's fine step per slice, one
+    # coarse step per slice; every problem's iterate K against the oracle's Parareal
+    ups = uneven_batch(24, seed=0)
+    K = 2
+    spec = []
+    for u in ups:
+        dtF = u.end_time / u.steps_fine
+        spec.append((u.n, u.slices, u.nu, u.c, 2 * u.slices * dtF, 2 * u.slices, u.slices))
+    probs = [sp.Problem(n, S, nu, c, T, sf, sc) for n, S, nu, c, T, sf, sc in spec]
+    b = sp.Batch(probs, kmax=K, coarse_dtype="f64", use_graphs=True)
+    u0, q = _uneven_fields(sp, b, ups)
+    b.run(u0, q)
+    for i, ((n, S, nu, c, T, sf, sc), u) in enumerate(zip(spec, ups)):
+        g = u.gauss
+        init = O.fill_gaussians(n, g.xc, g.yc, g.zc, g.sigma, g.amp)
+        oc = O.OConvection(n, n, n, 1.0 / n, nu, 0.0, *c)
+        out, _ = oc.parareal(init, S, K, T, T / sf, sf // S, T / sc, sc // S, "rk4", keep_iters=False)
+        e = relerr(q[i].get(), out)
+        assert e <= 1e-12, (i, n, S, e)
