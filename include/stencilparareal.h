@@ -152,6 +152,18 @@ int sp_do_euler_timestep(sp_prop* p, const sp_layout* lin, const void* in, const
 int sp_do_rk4_ex(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
                  double tStart, double dt, int steps);
 
+/* DoRK4 that allocates nothing (no counterpart in the reference): for callers that cannot
+ * call hipMalloc -- code under hipGraph capture, tight serving loops, harnesses that forbid
+ * allocation.  Same arguments and result as sp_do_rk4 (bit for bit) on SP_F64 fields with
+ * the fused kernels; the one intermediate field of a multi-step or in-place run lives in a
+ * device array of the library (2^22 points, 32 MiB per device, held until the process
+ * ends).  Fields of more than 4194304 points, the reference-order path
+ * (SP_PATH_STAGED) and k extents below 8 return SP_ERR_UNSUPPORTED.  Every call on one
+ * device shares that array: calls that overlap on a device (other streams or threads)
+ * race. */
+int sp_do_rk4_noalloc(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
+                      double tStart, double dt, int steps);
+
 /* Batched DoRK4 over `nslices` independent fields of one layout, slice b at
  * in + b*slice_stride (elements), all with the same start time; one fused kernel
  * launch per step covers every slice.  in != out required (no aliasing). */

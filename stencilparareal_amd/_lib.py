@@ -34,6 +34,7 @@ EXPORTS = [
     "sp_layout_compact", "sp_layout_stella_cuda", "sp_layout_storage_elems", "sp_prop_create",
     "sp_prop_destroy", "sp_prop_set_upwind", "sp_prop_set_path", "sp_prop_set_tile", "sp_prop_get_tile", "sp_prop_autotune", "sp_do_rk4",
     "sp_do_euler", "sp_do_rk4_timestep", "sp_do_euler_timestep", "sp_do_rk4_ex", "sp_do_rk4_batched",
+    "sp_do_rk4_noalloc",
     "sp_periodic_fill",
     "sp_parareal_update", "sp_norms", "sp_fill_exact", "sp_fill_gaussians", "sp_parareal_create",
     "sp_parareal_destroy", "sp_parareal_begin", "sp_parareal_iterate", "sp_parareal_iteration_times",
@@ -122,6 +123,7 @@ _SIGS = {
     "sp_prop_get_tile": (C.c_int, [_vp, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 5),
     "sp_prop_autotune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "sp_do_rk4": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
+    "sp_do_rk4_noalloc": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
     "sp_do_euler": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double, C.c_int]),
     "sp_do_rk4_timestep": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double]),
     "sp_do_euler_timestep": (C.c_int, [_vp, _lp, _vp, _lp, _vp, C.c_double, C.c_double]),

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -47,6 +48,33 @@ int check_layout(const sp_layout* l, const char* what)
 
 using namespace sp;
 
+// Scratch of sp_do_rk4_noalloc: the one intermediate field of a multi-step or in-place run,
+// in a file-scope device array instead of a library allocation.  Every device the library
+// runs on holds its own copy from code-object load until the process ends; calls on one
+// device share it (and keep its last contents), so calls that overlap on a device race.
+constexpr long long kNoAllocElems = 1LL << 22;  // 4194304 points: FP64 fields up to 160^3
+static __device__ __align__(256) double scratch[kNoAllocElems];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};  // per device: the device address of `scratch`
+
+// With `device` current: looks up this device's copy of `scratch` once.  Host code uses only
+// this address; the array's name on the host is a stand-in, not a device address.
+int noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return SP_OK;  // entry point refuses later
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return SP_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return sp::fail(SP_ERR_HIP, std::string("scratch lookup: ") + hipGetErrorString(e));
+    g_noalloc_ptr[device] = ptr;
+    return SP_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -261,6 +289,7 @@ int sp_ctx_create(int device, sp_ctx** out)
     SP_HIP(hipGetDeviceCount(&n));
     SP_CHECK(device >= 0 && device < n, SP_ERR_INVALID, "sp_ctx_create: no such device");
     SP_HIP(hipSetDevice(device));
+    SP_TRY(noalloc_scratch_setup(device));
     sp_ctx* c = new sp_ctx;
     c->device = device;
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
@@ -515,6 +544,50 @@ int sp_do_rk4(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout*
               double dt, int steps)
 {
     return rk4_common(p, lin, in, lout, out, tStart, dt, steps, true, "sp_do_rk4");
+}
+
+// sp_do_rk4 without allocation: the fused kernels on FP64 fields, the intermediate field in
+// this device's `scratch` (so the launches are those of sp_do_rk4: bit-for-bit its result).
+int sp_do_rk4_noalloc(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out,
+                      double tStart, double dt, int steps)
+{
+    const char* name = "sp_do_rk4_noalloc";
+    SP_CHECK(p, SP_ERR_INVALID, std::string(name) + ": null propagator");
+    SP_TRY(check_prop_layout(p, lin, in, name));
+    SP_TRY(check_prop_layout(p, lout, out, name));
+    SP_CHECK(steps >= 0, SP_ERR_INVALID, std::string(name) + ": steps < 0");
+    SP_CHECK(std::isfinite(dt) && std::isfinite(tStart), SP_ERR_INVALID, std::string(name) + ": non-finite time");
+    SP_CHECK(lin->dtype == SP_F64 && lout->dtype == SP_F64, SP_ERR_INVALID,
+             std::string(name) + ": fields must be SP_F64");
+    SP_CHECK(p->path == SP_PATH_FUSED && p->nk >= kFusedMinK, SP_ERR_UNSUPPORTED,
+             std::string(name) + ": runs the fused kernels only (the reference-order path and k extents below " +
+                 std::to_string(kFusedMinK) + " need stage fields; use sp_do_rk4)");
+    const long long n = (long long)p->ni * p->nj * p->nk;
+    SP_CHECK(n <= kNoAllocElems, SP_ERR_UNSUPPORTED,
+             std::string(name) + ": a field of " + std::to_string(n) + " points exceeds the scratch of " +
+                 std::to_string(kNoAllocElems) + " points (use sp_do_rk4)");
+    const int dev = p->ctx->device;
+    SP_CHECK(dev >= 0 && dev < kNoAllocMaxDevices && g_noalloc_ptr[dev], SP_ERR_UNSUPPORTED,
+             std::string(name) + ": no scratch on device " + std::to_string(dev));
+    if (steps == 0) return SP_OK;  // Convection.cpp:296: out untouched
+    SP_HIP(hipSetDevice(dev));
+    hipStream_t s = p->ctx->stream;
+    const View vin = view_of(*lin, in), vout = view_of(*lout, out);
+    const bool aliased = same_origin(lin, in, lout, out);
+    const View A = cview(g_noalloc_ptr[dev], p->ni, p->nj);
+    const FusedTile tile = pick_tile(p, 1, 1);
+    // out and A alternate so the last step lands in out; in place (in == out), the first
+    // step must not write in, so an odd count ends in A and is copied to out
+    View src = vin;
+    for (int i = 0; i < steps; ++i) {
+        const int left = steps - 1 - i;
+        const View dst = aliased ? ((i % 2 == 0) ? A : vout) : ((left % 2 == 0) ? vout : A);
+        const double t = tStart + i * dt;
+        SP_TRY(launch_rk4_fused(s, fused_step(p, 1, dt, nullptr, computenu(p, t)), tile, SP_F64, src, SP_F64, dst));
+        src = dst;
+    }
+    if (aliased && steps % 2 == 1) SP_TRY(launch_convert(s, p->ni, p->nj, p->nk, 1, SP_F64, A, SP_F64, vout));
+    return SP_OK;
 }
 
 int sp_do_rk4_ex(sp_prop* p, const sp_layout* lin, const void* in, const sp_layout* lout, void* out, double tStart,
