@@ -256,7 +256,8 @@ def cfg4_solve(a, w, conf, conv, init):
     log("create Parareal (BF16 G, 2 sub-steps, tol 1e-6, graphs; G tile autotuned)")
     pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=w.coarse_dtype, substeps=w.substeps,
                      tol=w.tol, use_graphs=True, autotune=True)
-    log("serial FP64 fine solve (DoSerial)")
+    log("autotune the one-slice FP64 tile (DoSerial); serial FP64 fine solve")
+    conv.autotune(1)
     pr.q = serial
     torch.cuda.synchronize()
     t0 = time.perf_counter()

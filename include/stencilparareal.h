@@ -129,9 +129,14 @@ int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* 
  * times every compiled fused-kernel tile shape and k chunk for `nslices` slices of this
  * propagator's size with the given storage types, and keeps the fastest for later
  * sp_do_rk4* calls with that slice count and compute type (FP64 when both types are
- * SP_F64, else FP32), unless a tile is set with sp_prop_set_tile.  Synchronous; allocates
- * two scratch batches for the duration; refused while the stream is being captured.
- * *best_us (nullable) receives the fastest step time in microseconds. */
+ * SP_F64, else FP32), unless a tile is set with sp_prop_set_tile.  Each candidate runs for
+ * about 2 ms of timed launches and the two fastest are timed again.  Results are kept for
+ * the process per (device, sizes, slice count, storage types), so later propagators and
+ * Parareal objects of the same sizes reuse them without timing; with the environment
+ * variable SP_AUTOTUNE_CACHE set to a file path they are also read from and appended to
+ * that file.  Synchronous; allocates two scratch batches while it times (none when a kept
+ * result applies); refused while the stream is being captured.  *best_us
+ * (nullable) receives the fastest step time in microseconds. */
 int sp_prop_autotune(sp_prop* p, int nslices, int in_dtype, int out_dtype, double* best_us);
 
 /* DoRK4: `steps` RK4 steps of size dt from tStart; step i runs at tStart + i*dt.
