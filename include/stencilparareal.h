@@ -52,6 +52,7 @@ extern "C" {
 #define SP_KERNEL_SCALAR 0 /* one column per lane, any sizes and storage types */
 #define SP_KERNEL_PAIR 1   /* FP64, two x-adjacent columns per lane, 16-byte accesses */
 #define SP_KERNEL_QUAD 2   /* FP64, 2x2 columns per lane in stages 1-3 (32x32 tiles, 512 threads) */
+#define SP_KERNEL_SPLIT 3  /* FP64, two launches of a two-stage x-pair kernel through a work field */
 
 /*
  * Field layout.  Element (i,j,k) of the interior, 0 <= i < ni etc., lives at
@@ -115,7 +116,12 @@ int sp_prop_set_path(sp_prop* p, int path);
  * kchunk planes per block (0: whole k extent).  variant SP_KERNEL_AUTO restores the
  * built-in choice.  Compiled shapes: scalar {32x32x1024, 32x24x1024, 24x24x1024,
  * 32x16x512, 16x16x256}, pair {32x32x512, 32x16x512, 16x16x256}, quad {32x32x512}
- * (the built-in choice for FP64 launches of 256 or more blocks). */
+ * (the built-in choice for FP64 launches of 256 or more blocks), split {32x32x512,
+ * 32x16x512, 32x16x256, 16x16x256}.  The split variant (FP64 fields, even ni) runs each
+ * step as two launches through a work field the propagator allocates (8 bytes per point
+ * of the batch) when the tile is chosen; its results equal the other FP64 variants' bit
+ * for bit.  Grouped batches (sp_batch_*) and sp_do_rk4_noalloc run the built-in FP64
+ * choice in its place. */
 int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int kchunk);
 /* The fused-kernel tile sp_do_rk4 (nslices = 1) or sp_do_rk4_batched (nslices > 1) will
  * use for FP64 compute (f64 = 1) or reduced-precision storage (f64 = 0): the one set with

@@ -37,13 +37,14 @@ class Convection:
         L.call("sp_prop_set_path", self.h, {"fused": L.SP_PATH_FUSED, "staged": L.SP_PATH_STAGED}[path])
 
     def set_tile(self, tx=0, ty=0, nthreads=0, kchunk=0, variant=None):
-        """Fused-kernel tile.  variant: "scalar", "pair", "quad" (FP64 32x32x512 only) or None
-        (built-in choice when tx == 0, else scalar)."""
+        """Fused-kernel tile.  variant: "scalar", "pair", "quad" (FP64 32x32x512 only), "split"
+        (FP64, two launches of a two-stage kernel) or None (built-in choice when tx == 0, else
+        scalar)."""
         if tx == 0 and variant is None:
             v = L.SP_KERNEL_AUTO
         else:
             v = {"scalar": L.SP_KERNEL_SCALAR, "pair": L.SP_KERNEL_PAIR, "quad": L.SP_KERNEL_QUAD,
-                 None: L.SP_KERNEL_SCALAR}[variant]
+                 "split": L.SP_KERNEL_SPLIT, None: L.SP_KERNEL_SCALAR}[variant]
         L.call("sp_prop_set_tile", self.h, v, tx, ty, nthreads, kchunk)
 
     def get_tile(self, nslices: int = 1, f64: bool = True) -> dict:
@@ -51,7 +52,8 @@ class Convection:
         v = [C.c_int() for _ in range(5)]
         L.call("sp_prop_get_tile", self.h, nslices, 1 if f64 else 0, *[C.byref(x) for x in v])
         var, tx, ty, nt, kc = (x.value for x in v)
-        names = {L.SP_KERNEL_SCALAR: "scalar", L.SP_KERNEL_PAIR: "pair", L.SP_KERNEL_QUAD: "quad"}
+        names = {L.SP_KERNEL_SCALAR: "scalar", L.SP_KERNEL_PAIR: "pair", L.SP_KERNEL_QUAD: "quad",
+                 L.SP_KERNEL_SPLIT: "split"}
         return {"variant": names.get(var, var), "tx": tx,
                 "ty": ty, "nthreads": nt, "kchunk": kc}
 

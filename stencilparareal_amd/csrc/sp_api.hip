@@ -141,7 +141,17 @@ FusedStep fused_step(const sp_prop* p, int nslices, double dt, const void* coef_
     st.dx = p->dx;
     st.coef_dev = coef_dev;
     st.nu = nu;
+    st.work = p->split.p;
+    st.work_bytes = p->split.cap;
     return st;
+}
+
+// The work field a split tile needs for `nslices` slices; nothing for the other variants.
+// Called where a tile is picked, before any launch is captured.
+int ensure_split(sp_prop* p, const FusedTile& t, int nslices)
+{
+    if (t.variant != SP_KERNEL_SPLIT) return SP_OK;
+    return p->split.ensure(split_work_bytes(p->ni, p->nj, p->nk, nslices));
 }
 
 // Autotuner: times every candidate tile / k chunk of the fused kernel for `nslices` slices
@@ -239,6 +249,7 @@ int tune_fused(sp_prop* p, int nslices, int idt, int odt, View in, View out, dou
     if (tune_memo_hit(p, nslices, idt, odt, best_us)) return SP_OK;
     std::vector<FusedTile> cand;
     fused_candidates(p->ni, p->nj, p->nk, nslices, f64, cand);
+    for (const FusedTile& t : cand) SP_TRY(ensure_split(p, t, nslices));
     hipEvent_t e0, e1;
     SP_HIP(hipEventCreate(&e0));
     SP_HIP(hipEventCreate(&e1));
@@ -284,6 +295,7 @@ int rk4_fused_run(sp_prop* p, int idt, View vin, int odt, View vout, bool aliase
     SP_TRY(p->scratch[0].ensure(bytes));
     if (aliased) SP_TRY(p->scratch[1].ensure(bytes));
     const FusedTile tile = pick_tile(p, 1, idt == SP_F64 && odt == SP_F64);
+    SP_TRY(ensure_split(p, tile, 1));
     const View A = cview(p->scratch[0].p, p->ni, p->nj), B = cview(p->scratch[1].p, p->ni, p->nj);
     View src = vin;
     int sdtype = idt;
@@ -510,7 +522,8 @@ static void prop_release(sp_prop* p)
     if (!p || --p->refs > 0) return;
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
-    for (Buf* b : {&p->scratch[0], &p->scratch[1], &p->qI, &p->k1, &p->k2, &p->k3, &p->k4, &p->nu_tab}) b->release();
+    for (Buf* b : {&p->scratch[0], &p->scratch[1], &p->split, &p->qI, &p->k1, &p->k2, &p->k3, &p->k4, &p->nu_tab})
+        b->release();
     sp_ctx* c = p->ctx;
     delete p;
     ctx_release(c);
@@ -546,8 +559,9 @@ int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int 
         p->tile = FusedTile{0, 0, 0, kchunk > 0 ? kchunk : -1, SP_KERNEL_AUTO};
         return SP_OK;
     }
-    SP_CHECK(variant == SP_KERNEL_SCALAR || variant == SP_KERNEL_PAIR || variant == SP_KERNEL_QUAD, SP_ERR_INVALID,
-             "sp_prop_set_tile: unknown variant");
+    SP_CHECK(variant == SP_KERNEL_SCALAR || variant == SP_KERNEL_PAIR || variant == SP_KERNEL_QUAD ||
+                 variant == SP_KERNEL_SPLIT,
+             SP_ERR_INVALID, "sp_prop_set_tile: unknown variant");
     const FusedTile t{tx, ty, nthreads, kchunk, variant};
     SP_CHECK(fused_tile_supported(t, 1), SP_ERR_UNSUPPORTED, "sp_prop_set_tile: tile shape not compiled");
     SP_CHECK(kchunk >= 0, SP_ERR_INVALID, "sp_prop_set_tile: negative kchunk");
@@ -656,7 +670,10 @@ int sp_do_rk4_noalloc(sp_prop* p, const sp_layout* lin, const void* in, const sp
     const View vin = view_of(*lin, in), vout = view_of(*lout, out);
     const bool aliased = same_origin(lin, in, lout, out);
     const View A = cview(g_noalloc_ptr[dev], p->ni, p->nj);
-    const FusedTile tile = pick_tile(p, 1, 1);
+    FusedTile tile = pick_tile(p, 1, 1);
+    // the split steps need a second intermediate field: the built-in one-launch tile instead
+    // (the same result bit for bit)
+    if (tile.variant == SP_KERNEL_SPLIT) tile = default_tile(p->ni, p->nj, p->nk, 1, 1);
     // out and A alternate so the last step lands in out; in place (in == out), the first
     // step must not write in, so an odd count ends in A and is copied to out
     View src = vin;
@@ -728,6 +745,7 @@ int sp_do_rk4_batched(sp_prop* p, int nslices, const sp_layout* lin, const void*
         A.p = (char*)p->scratch[0].p + lout->origin * dtype_size(lout->dtype);
     }
     const FusedTile tile = pick_tile(p, nslices, lin->dtype == SP_F64 && lout->dtype == SP_F64);
+    SP_TRY(ensure_split(p, tile, nslices));
     View cur = vin;
     int cdt = lin->dtype;
     for (int i = 0; i < steps; ++i) {
@@ -875,6 +893,7 @@ struct sp_parareal {
     size_t n = 0;                 // points per slice
     Buf U, F, FS, Gold, Gw0, Gw1, Q, coefF, coefG, part;
     Buf D;  // difference form: U_n^{k+1} - U_n^k of the chain's current slice
+    Buf W;  // work field of split-tile steps (its own: captured graphs keep its address)
     std::vector<double> nuF_host, nuG_host;
     bool nu_const = true;
     bool begun = false;
@@ -901,6 +920,15 @@ View batch_view(void* p, const sp_parareal* pr, int dt, int slice0 = 0)
     View v = cview((char*)p + (size_t)slice0 * pr->n * dtype_size(dt), pr->p->ni, pr->p->nj);
     v.slice = (long long)pr->n;
     return v;
+}
+
+// A fused step of this Parareal's propagator with the Parareal's own split work field.
+FusedStep pr_step(const sp_parareal* pr, int nslices, double dt, const void* nud, double nu)
+{
+    FusedStep st = fused_step(pr->p, nslices, dt, nud, nu);
+    st.work = pr->W.p;
+    st.work_bytes = pr->W.cap;
+    return st;
 }
 
 // Coarse propagation of `nslices` batched slices, tsG sub-steps from per-slice start
@@ -945,7 +973,7 @@ int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View
         }
         if (pr->cfg.coarse_kind == SP_COARSE_RK4) {
             const FusedTile& tile = nslices == 1 ? pr->tileG1 : pr->tileGS;
-            SP_TRY(launch_rk4_fused(s, fused_step(p, nslices, pr->dtG, nud, nu), tile, cdt, cur, pr->gdt, d));
+            SP_TRY(launch_rk4_fused(s, pr_step(pr, nslices, pr->dtG, nud, nu), tile, cdt, cur, pr->gdt, d));
         } else {
             if (nud) return fail(SP_ERR_UNSUPPORTED, "batched Euler coarse with time-dependent nu");
             const StagedParams P{p->ni, p->nj, p->nk, p->cx, p->cy, p->cz, p->dx, p->dx * p->dx};
@@ -985,7 +1013,7 @@ int fine_sweep(sp_parareal* pr)
         const int left = pr->tsF - 1 - i;
         const View d = (left % 2 == 0) ? F : FS;
         const void* nud = pr->nu_const ? nullptr : (const char*)pr->coefF.p + (size_t)i * pr->S * coefs_size();
-        SP_TRY(launch_rk4_fused(s, fused_step(p, pr->S, pr->dtF, nud, p->nu0), pr->tileF, SP_F64, cur, SP_F64, d));
+        SP_TRY(launch_rk4_fused(s, pr_step(pr, pr->S, pr->dtF, nud, p->nu0), pr->tileF, SP_F64, cur, SP_F64, d));
         cur = d;
     }
     return SP_OK;
@@ -1182,6 +1210,19 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
     pr->tileF = pick_tile(p, pr->S, 1);
     pr->tileG1 = pick_tile(p, 1, pr->gdt == SP_F64);
     pr->tileGS = pick_tile(p, pr->S, pr->gdt == SP_F64);
+    {
+        size_t wb = 0;
+        if (pr->tileF.variant == SP_KERNEL_SPLIT) wb = split_work_bytes(p->ni, p->nj, p->nk, pr->S);
+        if (pr->gdt == SP_F64 && pr->tileG1.variant == SP_KERNEL_SPLIT)
+            wb = std::max(wb, split_work_bytes(p->ni, p->nj, p->nk, 1));
+        if (pr->gdt == SP_F64 && pr->tileGS.variant == SP_KERNEL_SPLIT)
+            wb = std::max(wb, split_work_bytes(p->ni, p->nj, p->nk, pr->S));
+        const int r = wb ? pr->W.ensure(wb) : SP_OK;
+        if (r != SP_OK) {
+            sp_parareal_destroy(pr);
+            return r;
+        }
+    }
     pr->part_host.resize((size_t)pr->S * kReduceBlocks * 2);
     *out = pr;
     return SP_OK;
@@ -1200,7 +1241,7 @@ int sp_parareal_destroy(sp_parareal* pr)
     for (hipEvent_t e : pr->evU)
         if (e) (void)hipEventDestroy(e);
     for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->coefF, &pr->coefG, &pr->part,
-                   &pr->D})
+                   &pr->D, &pr->W})
         b->release();
     sp_prop* p = pr->p;
     delete pr;

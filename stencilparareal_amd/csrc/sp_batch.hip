@@ -392,6 +392,8 @@ int sp_batch_create(sp_ctx* ctx, int nprob, const sp_problem* probs, const sp_ba
             int v = 0;
             chk(sp_prop_get_tile(q.prop, nslices, f64, &v, &t.tx, &t.ty, &t.nt, &t.kchunk));
             t.variant = v;
+            // grouped launches have no split form: the built-in one-launch tile
+            if (t.variant == SP_KERNEL_SPLIT) t = default_tile(q.d.n, q.d.n, q.d.n, nslices, f64);
             tuned[key] = t;
         };
         if (shared_fine)

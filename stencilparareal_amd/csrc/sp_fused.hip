@@ -781,6 +781,197 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
     }
 }
 
+// Waits for every outstanding vector-memory access of this wave (s_waitcnt vmcnt(0), the
+// other counters left alone).  The compiler's own waits treat loads and stores as completing
+// out of order, so with a store pending every use of a loaded register waits for all loads
+// issued so far, the prefetches included; one explicit wait where all the data a stretch of
+// planes consumes is known to have been requested lets the loads issued after it stay in flight.
+__device__ __forceinline__ void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// ---- FP64 RK4 step in two launches of a two-stage kernel ----------------------------------
+// The four-stage stream fills its pipeline with 19 planes per block.  A slice small enough to
+// stay in the last-level cache (one 128^3 slice: 16.8 MB) is chunked in k to give every CU
+// a block, and then a block streams 35 planes for 16 outputs, its stage-1 region 2.4x its
+// tile.  Split in two launches, each block streams kchunk + 9 planes over the regions of
+// stages 3 and 4 (tile + 4, tile):
+//   launch A: u = q,  v_a = q + f1 L u,  out = q + f2 L v_a   (= v2, into a work field)
+//   launch B: u = v2, v_a = q + f3 L u,  out = q + f4 L v_a   (= q')
+// The work field adds one write and two reads of a field per step, cheap while the fields
+// stay cached.  Each stage is pair_step with the same operands in the same order as in the
+// four-stage kernel, so q' equals that kernel's bit for bit.
+// Schedule, iteration t: stage a reads u plane t and emits v_a(t-2) into LDS; stage b reads
+// v_a(t-3) and emits out(t-5).  q for the updates: in launch A stage a takes it from the
+// centres it read two planes before; otherwise it is loaded from the q field.
+struct HalfArgs {
+    FusedArgs a;       // a.in: u; a.out: the output
+    View q;            // the update base (launch A: the same field as a.in)
+    unsigned q_bytes;  // bytes one slice of q spans from its origin
+};
+
+template <int TX, int TY, int NT, bool FIRST>
+__device__ __forceinline__ void rk4_half_pair_body(const HalfArgs& h, const int bid)
+{
+    const FusedArgs& a = h.a;
+    using G3 = Geo<TX, TY, 3>;
+    constexpr int A0P = G3::AI / 2;  // pairs of the u plane incl. its 4-cell halo
+    constexpr int M0 = (A0P + NT - 1) / NT;
+    using PMa = PairMap<TX, TY, 3, NT>;
+    using PMb = PairMap<TX, TY, 4, NT>;
+    constexpr int Ma = PMa::M, Mb = PMb::M;
+
+    __shared__ __attribute__((aligned(16))) double P0[2][G3::AI];
+    __shared__ __attribute__((aligned(16))) double Pa[2][Geo<TX, TY, 4>::AI];
+
+    const int tid = threadIdx.x;
+    const int ni = a.ni, nj = a.nj, nk = a.nk;
+    const int tile = bid % a.tiles;
+    const int chunk = (bid / a.tiles) % a.chunks;
+    const int slice = bid / (a.tiles * a.chunks);
+    const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
+    const int z0 = chunk * a.kchunk;
+    const int zend = min(z0 + a.kchunk, nk);
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc((const double*)a.in.p + slice * a.in.slice, a.in_bytes);
+    const __amdgpu_buffer_rsrc_t rq = make_rsrc((const double*)h.q.p + slice * h.q.slice, h.q_bytes);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc((double*)a.out.p + slice * a.out.slice, a.out_bytes);
+    const int iskb = (int)a.in.sk * 8, qskb = (int)h.q.sk * 8, oskb = (int)a.out.sk * 8;
+    const int isj = (int)a.in.sj, qsj = (int)h.q.sj, osj = (int)a.out.sj;
+    const StarW<double> w = a.cdev ? load_w<double>(a.cdev[slice]) : load_w<double>(a.c);
+    const double fa = FIRST ? w.f1 : w.f3, fb = FIRST ? w.f2 : w.f4;
+
+    int off0[M0];
+#pragma unroll
+    for (int m = 0; m < M0; ++m) {
+        const int g = min(tid + m * NT, A0P - 1);
+        const int ax = 2 * (g % (G3::WI / 2)), by = g / (G3::WI / 2);
+        off0[m] = (pmod(x0 - 4 + ax, ni) + pmod(y0 - 4 + by, nj) * isj) * 8;
+    }
+    int qa[Ma], qb[Mb], ob[Mb];
+    if constexpr (!FIRST) pair_offsets<TX, TY, 3, NT, Ma>(qa, x0, y0, ni, nj, qsj);
+    pair_offsets<TX, TY, 4, NT, Mb>(qb, x0, y0, ni, nj, qsj);
+#pragma unroll
+    for (int m = 0; m < Mb; ++m) {
+        int ax, by;
+        PMb::at(tid + m * NT, ax, by);
+        ob[m] = (pmod(x0 + ax, ni) + pmod(y0 + by, nj) * osj) * 8;  // periodic image for partial tiles
+    }
+    constexpr bool MASK = TX * TY <= 512;
+    bool lva[Ma], lvb[Mb];
+    if constexpr (MASK) {
+#pragma unroll
+        for (int m = 0; m < Ma; ++m) lva[m] = PMa::live(tid + m * NT);
+#pragma unroll
+        for (int m = 0; m < Mb; ++m) lvb[m] = PMb::live(tid + m * NT);
+    }
+    const auto wr0 = [&](int m) { return !MASK || m < M0 - 1 || tid + m * NT < A0P; };
+
+    double acca[Ma][2][4], accb[Mb][2][4];
+    double da[Ma][2], db[Mb][2];
+    d2 ca[Ma], cb[Mb];
+    // Global loads are issued four planes before use, in rings of four slots indexed by
+    // (plane - tq0) & 3, and waited for once every four planes (vm_wait_all at phase 0):
+    // everything a plane consumes was loaded before that wait, so no other wait stalls.
+    d2 ra[2][Ma];     // launch A: q one and two planes back (ring by plane parity)
+    d2 pre[4][M0];    // u planes t+1 .. t+4
+    d2 qva[4][Ma];    // launch B: q planes t-2 .. t+1 for stage a
+    d2 qvb[4][Mb];    // q planes t-5 .. t-2 for stage b
+#pragma unroll
+    for (int m = 0; m < Ma; ++m) {
+        ra[0][m] = ra[1][m] = d2{0., 0.};
+        for (int e = 0; e < 2; ++e) acca[m][e][0] = acca[m][e][1] = acca[m][e][2] = acca[m][e][3] = 0.;
+    }
+#pragma unroll
+    for (int m = 0; m < Mb; ++m)
+        for (int e = 0; e < 2; ++e) accb[m][e][0] = accb[m][e][1] = accb[m][e][2] = accb[m][e][3] = 0.;
+
+    const int tq0 = z0 - 4, tq1 = zend + 4;  // u planes read: tq0 .. tq1 - 1
+    const int tend = zend + 5;
+    {
+        const int zo = wrapz(tq0, nk) * iskb;
+#pragma unroll
+        for (int m = 0; m < M0; ++m)
+            if (wr0(m)) *(d2*)&P0[tq0 & 1][2 * min(tid + m * NT, A0P - 1)] = ld2(rin, off0[m], zo);
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) {
+            const int zj = wrapz(tq0 + j, nk) * iskb;  // planes past tq1 - 1 are read and not used
+#pragma unroll
+            for (int m = 0; m < M0; ++m) pre[j & 3][m] = ld2(rin, off0[m], zj);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if constexpr (!FIRST) {
+                const int za = wrapz(tq0 - 2 + j, nk) * qskb;
+#pragma unroll
+                for (int m = 0; m < Ma; ++m) qva[(j + 2) & 3][m] = ld2(rq, qa[m], za);
+            }
+            const int zb = wrapz(tq0 + j, nk) * qskb;  // first used at tq0 + 5
+#pragma unroll
+            for (int m = 0; m < Mb; ++m) qvb[j][m] = ld2(rq, qb[m], zb);
+        }
+    }
+    __syncthreads();
+
+    auto plane = [&](auto Rc, int t) {
+        constexpr int R = decltype(Rc)::value;
+        constexpr int B = R & 1;
+        constexpr int S1 = (R + 1) & 3, SA = (R + 2) & 3, SB = (R + 3) & 3;  // slots of planes t+1, t-2, t-5
+        const int cur = t & 1, prv = cur ^ 1;
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (R == 0) vm_wait_all();
+        if (t + 1 < tq1) {
+#pragma unroll
+            for (int m = 0; m < M0; ++m)
+                if (wr0(m)) *(d2*)&P0[prv][2 * min(tid + m * NT, A0P - 1)] = pre[S1][m];
+            if (t + 5 < tq1) {
+                const int zo = wrapz(t + 5, nk) * iskb;
+#pragma unroll
+                for (int m = 0; m < M0; ++m) pre[S1][m] = ld2(rin, off0[m], zo);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (t < tq1) {
+            pair_step<TX, TY, 3, NT, Ma, R & 3>(P0[cur], w, acca, da, ca);
+#pragma unroll
+            for (int m = 0; m < Ma; ++m) {
+                const d2 q = FIRST ? ra[B][m] : qva[SA][m];
+                if (!MASK || lva[m])
+                    *(d2*)&Pa[cur][PMa::out(tid + m * NT)] = d2{fmaT(fa, da[m][0], q.x), fmaT(fa, da[m][1], q.y)};
+                if constexpr (FIRST) ra[B][m] = ca[m];
+            }
+            if constexpr (!FIRST) {
+                const int zo = wrapz(t + 2, nk) * qskb;
+#pragma unroll
+                for (int m = 0; m < Ma; ++m) qva[SA][m] = ld2(rq, qa[m], zo);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (t >= tq0 + 5) {
+            pair_step<TX, TY, 4, NT, Mb, (R + 1) & 3>(Pa[prv], w, accb, db, cb);
+            const int zp = t - 5;
+            if (zp >= z0 && zp < zend) {
+                const int zo = zp * oskb;
+#pragma unroll
+                for (int m = 0; m < Mb; ++m)
+                    if (!MASK || lvb[m])
+                        st2(rout, ob[m], zo,
+                            d2{fmaT(fb, db[m][0], qvb[SB][m].x), fmaT(fb, db[m][1], qvb[SB][m].y)});
+            }
+            const int zo = wrapz(t - 1, nk) * qskb;
+#pragma unroll
+            for (int m = 0; m < Mb; ++m) qvb[SB][m] = ld2(rq, qb[m], zo);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
+    for (int t = tq0; t < tend; t += 4) {
+        plane(std::integral_constant<int, 0>{}, t);
+        if (t + 1 < tend) plane(std::integral_constant<int, 1>{}, t + 1);
+        if (t + 2 < tend) plane(std::integral_constant<int, 2>{}, t + 2);
+        if (t + 3 < tend) plane(std::integral_constant<int, 3>{}, t + 3);
+    }
+}
+
 // ---- FP64 2x2 variant (32x32 tiles, 512 threads) -----------------------------------------
 // Stages 1-3 give each lane a 2x2 block of columns (an x pair in two consecutive rows).  The
 // centres of one row are y neighbours of the other, so 10 ds_read_b128 per block and stage
@@ -1123,6 +1314,12 @@ __global__ void __launch_bounds__(QuadCfg::NT) k_rk4_fused_quad(FusedArgs a)
     rk4_quad_body(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
+template <int TX, int TY, int NT, bool FIRST>
+__global__ void __launch_bounds__(NT) k_rk4_half_pair(HalfArgs h)
+{
+    rk4_half_pair_body<TX, TY, NT, FIRST>(h, xcd_block(blockIdx.x, gridDim.x));
+}
+
 // One launch over several independent batches (problems of different sizes, steps and
 // coefficients): entry e owns blocks [first[e], first[e+1]); a block finds its entry by a
 // scalar scan (entries are few) and runs the same body on that entry's arguments.
@@ -1270,6 +1467,40 @@ int launch_quad(hipStream_t s, const FusedStep& st, const FusedTile& tile, View 
     return SP_OK;
 }
 
+// Launch A (in -> work: v2) and launch B (work, in -> out: q').  Requires st.work.
+int launch_split(hipStream_t s, const FusedStep& st, const FusedTile& tile, View in, View out)
+{
+    SP_CHECK(st.work && st.work_bytes >= split_work_bytes(st.ni, st.nj, st.nk, st.nslices), SP_ERR_INVALID,
+             "fused RK4 (split): no work field for this batch");
+    View wv;
+    wv.p = st.work;
+    wv.sj = st.ni;
+    wv.sk = (long long)st.ni * st.nj;
+    wv.slice = wv.sk * st.nk;
+    HalfArgs A, B;
+    SP_TRY(fill_args(A.a, st, tile, in, wv, 8, 8));
+    A.q = in;
+    A.q_bytes = A.a.in_bytes;
+    SP_TRY(fill_args(B.a, st, tile, wv, out, 8, 8));
+    B.q = in;
+    B.q_bytes = A.a.in_bytes;
+    const dim3 grid(nblocks_of(A.a, st.nslices));
+#define SP_SPLIT_CASE(TX_, TY_, NT_)                                                           \
+    if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                                  \
+        hipLaunchKernelGGL((k_rk4_half_pair<TX_, TY_, NT_, true>), grid, dim3(NT_), 0, s, A);  \
+        SP_HIP(hipGetLastError());                                                             \
+        hipLaunchKernelGGL((k_rk4_half_pair<TX_, TY_, NT_, false>), grid, dim3(NT_), 0, s, B); \
+        SP_HIP(hipGetLastError());                                                             \
+        return SP_OK;                                                                          \
+    }
+    SP_SPLIT_CASE(32, 32, 512)
+    SP_SPLIT_CASE(32, 16, 512)
+    SP_SPLIT_CASE(32, 16, 256)
+    SP_SPLIT_CASE(16, 16, 256)
+#undef SP_SPLIT_CASE
+    return fail(SP_ERR_UNSUPPORTED, "fused RK4 (split): unsupported tile shape");
+}
+
 // 16-byte pair access needs even x extent, even strides and 16-byte aligned origins.
 bool pair_ok(const FusedStep& st, View in, View out)
 {
@@ -1354,7 +1585,13 @@ bool fused_tile_supported(const FusedTile& t, int /*compute_f64*/)
 {
     static const int scalar[][3] = {{32, 32, 1024}, {32, 24, 1024}, {24, 24, 1024}, {32, 16, 512}, {16, 16, 256}};
     static const int pair[][3] = {{32, 32, 512}, {32, 16, 512}, {16, 16, 256}};
+    static const int split[][3] = {{32, 32, 512}, {32, 16, 512}, {32, 16, 256}, {16, 16, 256}};
     if (t.variant == SP_KERNEL_QUAD) return t.tx == QuadCfg::TX && t.ty == QuadCfg::TY && t.nt == QuadCfg::NT;
+    if (t.variant == SP_KERNEL_SPLIT) {
+        for (const auto& sh : split)
+            if (t.tx == sh[0] && t.ty == sh[1] && t.nt == sh[2]) return true;
+        return false;
+    }
     if (t.variant == SP_KERNEL_PAIR) {
         for (const auto& sh : pair)
             if (t.tx == sh[0] && t.ty == sh[1] && t.nt == sh[2]) return true;
@@ -1400,21 +1637,27 @@ void fused_candidates(int ni, int nj, int nk, int nslices, int compute_f64, std:
 {
     static const int scalar[][3] = {{32, 32, 1024}, {32, 24, 1024}, {24, 24, 1024}, {32, 16, 512}, {16, 16, 256}};
     static const int pair[][3] = {{32, 32, 512}, {32, 16, 512}, {16, 16, 256}};
+    static const int split[][3] = {{32, 32, 512}, {32, 16, 512}, {32, 16, 256}, {16, 16, 256}};
     std::vector<FusedTile> shapes;
     if (compute_f64 && ni % 2 == 0) {
         shapes.push_back(FusedTile{QuadCfg::TX, QuadCfg::TY, QuadCfg::NT, 0, SP_KERNEL_QUAD});
         for (const auto& sh : pair) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_PAIR});
+        // the split steps move a third field: candidates while a field of the batch stays
+        // within 64 MiB (the last-level cache holds the three)
+        if (split_work_bytes(ni, nj, nk, nslices) <= (64u << 20))
+            for (const auto& sh : split) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_SPLIT});
     }
     for (const auto& sh : scalar) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_SCALAR});
     out.clear();
     for (const FusedTile& t : shapes) {
         const long long blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
-        // k chunks: the whole extent, and shorter ones (>= 16 planes) only while the tiles
-        // alone give fewer than 4 blocks per CU
+        // k chunks: the whole extent, and shorter ones (>= 16 planes, >= 8 for the split
+        // kernel's 9-plane fill) only while the tiles alone give fewer than 4 blocks per CU
+        const int cmin = t.variant == SP_KERNEL_SPLIT ? 8 : 16;
         std::vector<int> kcs{0};
         if (blocks < 1024)
-            for (int c : {nk / 2, nk / 4, nk / 8, 64, 32, 16})
-                if (c >= 16 && c < nk && std::find(kcs.begin(), kcs.end(), c) == kcs.end()) kcs.push_back(c);
+            for (int c : {nk / 2, nk / 4, nk / 8, 64, 32, 16, 8})
+                if (c >= cmin && c < nk && std::find(kcs.begin(), kcs.end(), c) == kcs.end()) kcs.push_back(c);
         for (int c : kcs) {
             FusedTile x = t;
             x.kchunk = c;
@@ -1433,7 +1676,8 @@ int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile_i
                                             std::to_string(kFusedMinK) + " (use the staged path)");
     if (f64 && tile.variant == SP_KERNEL_QUAD && pair_ok(st, in, out)) return launch_quad(s, st, tile, in, out);
     if (f64 && tile.variant == SP_KERNEL_PAIR && pair_ok(st, in, out)) return launch_pair(s, st, tile, in, out);
-    if (tile.variant == SP_KERNEL_PAIR || tile.variant == SP_KERNEL_QUAD) {
+    if (f64 && tile.variant == SP_KERNEL_SPLIT && pair_ok(st, in, out)) return launch_split(s, st, tile, in, out);
+    if (tile.variant == SP_KERNEL_PAIR || tile.variant == SP_KERNEL_QUAD || tile.variant == SP_KERNEL_SPLIT) {
         // odd sizes, unaligned views or reduced-precision storage: the scalar kernel with
         // its fastest FP64 shape and the caller's k chunk
         tile.tx = 32;

@@ -52,6 +52,7 @@ struct sp_prop {
     int path = SP_PATH_FUSED;
     sp::FusedTile tile{0, 0, 0, -1, SP_KERNEL_AUTO};  // tx == 0: automatic
     Buf scratch[2];                // ping-pong for the fused steppers
+    Buf split;                     // work field of SP_KERNEL_SPLIT steps (FP64, compact slices)
     Buf qI, k1, k2, k3, k4;         // staged path temporaries (f64, compact)
     Buf nu_tab;                     // per-step nu table
     std::vector<double> nu_host;

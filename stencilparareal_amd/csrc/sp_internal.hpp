@@ -136,7 +136,7 @@ int launch_euler_fused(hipStream_t s, const StagedParams& P, View qin, View qout
 // Fused RK4 step over `nslices` batched slices.  Types by dtype codes.
 struct FusedTile {
     int tx, ty, nt, kchunk;
-    int variant;  // SP_KERNEL_SCALAR, SP_KERNEL_PAIR or SP_KERNEL_QUAD
+    int variant;  // SP_KERNEL_SCALAR, SP_KERNEL_PAIR, SP_KERNEL_QUAD or SP_KERNEL_SPLIT
 };
 struct FusedStep {
     int ni, nj, nk;
@@ -145,7 +145,14 @@ struct FusedStep {
     double cx, cy, cz, dx;
     const void* coef_dev;  // per-slice coefficient records for this step (device) or nullptr
     double nu;             // used when coef_dev == nullptr
+    void* work = nullptr;  // SP_KERNEL_SPLIT: FP64 field of nslices compact slices (v2)
+    size_t work_bytes = 0;
 };
+// Bytes of the work field a SP_KERNEL_SPLIT step over `nslices` slices needs.
+inline size_t split_work_bytes(int ni, int nj, int nk, int nslices)
+{
+    return (size_t)ni * nj * nk * nslices * 8;
+}
 // Coefficient record of one fused RK4 step (stencil weights and stage factors for dt).
 size_t coefs_size();
 void fill_coefs(void* dst, double nu, double cx, double cy, double cz, double dx, double h);

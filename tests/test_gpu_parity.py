@@ -92,7 +92,8 @@ def test_euler_bitwise(sp, upwind, cvec):
 
 
 TILES = [("scalar", 32, 32, 1024), ("scalar", 24, 24, 1024), ("scalar", 32, 16, 512), ("scalar", 16, 16, 256),
-         ("pair", 32, 32, 512), ("pair", 32, 16, 512), ("pair", 16, 16, 256), ("quad", 32, 32, 512)]
+         ("pair", 32, 32, 512), ("pair", 32, 16, 512), ("pair", 16, 16, 256), ("quad", 32, 32, 512),
+         ("split", 32, 32, 512), ("split", 32, 16, 512), ("split", 32, 16, 256), ("split", 16, 16, 256)]
 
 
 @pytest.mark.parametrize("N", [8, 16, 20, 32, 48, 64])
@@ -109,7 +110,7 @@ def test_fused_rk4_vs_oracle(sp, N, tile):
     assert relerr(b.get(), ref) <= 1e-12
 
 
-@pytest.mark.parametrize("variant", ["scalar", "pair", "quad"])
+@pytest.mark.parametrize("variant", ["scalar", "pair", "quad", "split"])
 @pytest.mark.parametrize("kchunk", [0, 7, 16])
 def test_fused_rk4_kchunks_and_nonsquare(sp, kchunk, variant):
     ni, nj, nk = 40, 24, 36
@@ -123,7 +124,7 @@ def test_fused_rk4_kchunks_and_nonsquare(sp, kchunk, variant):
     assert relerr(b.get(), ref) <= 1e-12
 
 
-@pytest.mark.parametrize("variant", ["pair", "quad"])
+@pytest.mark.parametrize("variant", ["pair", "quad", "split"])
 @pytest.mark.parametrize("shape", [(40, 27, 20), (34, 9, 11), (64, 33, 40)])
 def test_fused_rk4_odd_y_extent_and_reference_layout(sp, shape, variant):
     # odd nj: block rows of the 2x2 kernel straddle the periodic wrap in y; partial tiles
@@ -139,11 +140,14 @@ def test_fused_rk4_odd_y_extent_and_reference_layout(sp, shape, variant):
     assert relerr(b.get(), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("variant", [None, "split"])
 @pytest.mark.parametrize("steps", [0, 1, 2, 3])
-def test_fused_rk4_aliasing_and_zero_steps(sp, steps):
+def test_fused_rk4_aliasing_and_zero_steps(sp, steps, variant):
     N = 16
     q0 = smooth_field(N, 7)
     conv = sp.Convection(N, N, N, 1. / N, 0.01, 30.0, *C3)
+    if variant:
+        conv.set_tile(16, 16, 256, 8, variant=variant)
     a = conv.field()
     a.set(q0)
     conv.DoRK4(a, a, 0.1, 1e-3, steps)
@@ -157,7 +161,8 @@ def test_pair_quad_and_scalar_kernels_agree_bitwise(sp):
     q0 = rand_field((N, N, N), 9)
     outs = []
     for variant, tile in (("scalar", (32, 32, 1024)), ("pair", (32, 32, 512)), ("pair", (32, 16, 512)),
-                          ("quad", (32, 32, 512))):
+                          ("quad", (32, 32, 512)), ("split", (32, 32, 512)), ("split", (32, 16, 256)),
+                          ("split", (16, 16, 256))):
         conv = sp.Convection(N, N, N, 1. / N, 0.01, 0.0, *C3)
         conv.set_tile(*tile, kchunk=16, variant=variant)
         a, b = conv.field(), conv.field()
@@ -168,7 +173,7 @@ def test_pair_quad_and_scalar_kernels_agree_bitwise(sp):
         assert np.array_equal(outs[0], o)
 
 
-@pytest.mark.parametrize("variant", ["pair", "quad"])
+@pytest.mark.parametrize("variant", ["pair", "quad", "split"])
 def test_odd_sizes_fall_back_to_scalar(sp, variant):
     ni, nj, nk = 21, 18, 17
     q0 = rand_field((nk, nj, ni), 6)
@@ -208,7 +213,7 @@ def test_autotuned_tile_is_used_and_exact(sp, nslices, dtype):
     us = conv.autotune(nslices, dtype, dtype)
     assert 0 < us < 1e6
     t = conv.get_tile(nslices, dtype == "f64")
-    assert t["variant"] in ("scalar", "pair", "quad") and t["tx"] > 0 and t["kchunk"] >= 0
+    assert t["variant"] in ("scalar", "pair", "quad", "split") and t["tx"] > 0 and t["kchunk"] >= 0
     if dtype == "f64":
         q0 = smooth_field(N, 11)
         a, b = conv.field(), conv.field()
@@ -427,6 +432,30 @@ def test_parareal_fused_vs_oracle(sp, coarse, graphs):
     # DoSerial
     pr.DoSerial()
     assert relerr(sol.get(), serial) <= 1e-12
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_parareal_split_tile_bitwise_and_vs_oracle(sp, graphs):
+    # every fine and coarse step as two launches through the Parareal's own work field
+    # (batched over the slices in the fine sweep, one slice in the chain): the iterates
+    # equal the one-launch kernels' bit for bit, and the oracle's to 1e-12
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4", N=16, nufreq=80.0)
+    outs = []
+    for variant in (None, "split"):
+        if variant:
+            conv.set_tile(16, 16, 256, 8, variant=variant)
+        init, sol = conv.field(), conv.field()
+        init.set(u0)
+        pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", use_graphs=graphs)
+        rep = pr.DoParallel()
+        assert rep.iterations == conf.kmax
+        outs.append(sol.get())
+        pr.DoSerial()
+        outs.append(sol.get())
+    assert conv.get_tile(1)["variant"] == "split"
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
+    assert relerr(outs[2], iters[conf.kmax]) <= 1e-12
+    assert relerr(outs[3], serial) <= 1e-12
 
 
 def test_parareal_autotuned_vs_oracle(sp):

@@ -87,3 +87,19 @@ def test_noalloc_refuses_what_it_cannot_hold(sp):
     with pytest.raises(L.SPError) as ei:
         rk4(L, "sp_do_rk4_noalloc", staged, a, b, 2)
     assert ei.value.code == L.SP_ERR_UNSUPPORTED
+
+
+def test_noalloc_with_split_tile_matches_rk4(sp):
+    # a split tile set on the propagator: sp_do_rk4 runs two launches per step through its
+    # work field, sp_do_rk4_noalloc the built-in one-launch tile; the results agree bit for bit
+    from stencilparareal_amd import _lib as L
+    n = 32
+    conv = sp.Convection(n, n, n, 1.0 / n, 5e-3, 30.0, 1.0, 0.5, 0.25)
+    conv.set_tile(32, 16, 256, 8, variant="split")
+    u = np.random.default_rng(5).standard_normal((n, n, n))
+    for steps in (1, 4):
+        a, r, g = conv.field("a"), conv.field("r"), conv.field("g")
+        a.set(u)
+        rk4(L, "sp_do_rk4", conv, a, r, steps)
+        rk4(L, "sp_do_rk4_noalloc", conv, a, g, steps)
+        assert np.array_equal(g.get(), r.get()), steps
