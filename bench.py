@@ -449,7 +449,7 @@ def roofline_only(a):
     from stencilparareal_amd.config import workload
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     w = workload(4 if a.workload == "cfg4" else 2)
-    N, S = w.N, w.slices
+    N, S = w.N, a.slices or w.slices
     conf = conf_of(w)
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
     if a.variant == "auto":
@@ -490,6 +490,9 @@ def main():
     ap.add_argument("--reps", type=int, default=5, help="launches of the kernel with --roofline-only")
     ap.add_argument("--variant", choices=["auto", "quad", "pair"], default="auto",
                     help="fused kernel for --roofline-only: autotuned (default) or a fixed 32x32x512 variant")
+    ap.add_argument("--slices", type=int, default=0,
+                    help="slices per launch with --roofline-only (default: the workload's; 1: the "
+                         "single-slice step of the coarse chain and the serial solve)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the time slices are the batch "
