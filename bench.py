@@ -245,29 +245,15 @@ def cfg4(a):
     print(json.dumps(out), flush=True)
 
 
-def cfg4_solve(a, w, conf, conv, init):
-    """The once-timed runs beside the steps: serial fine, the tolerance-stopped Parareal solve
-    and the plain comparator estimate."""
+def parareal_solve(conv, init, conf, w, sol, serial, gdt):
+    """One tolerance-stopped Parareal solve with G in storage type `gdt`, driven iteration by
+    iteration (DoParallel, Parareal.h:110-171: begin = the initial coarse chain, iterations
+    until the increment test, the last slice's state), each iteration logged."""
     import torch
     import stencilparareal_amd as sp
-    N, S = w.N, w.slices
-    tsF, dtF = conf.timeStepsFinePerTimeSlice(), conf.dtFine()
-    sol, serial = conv.field("q"), conv.field("qreference")
-    log("create Parareal (BF16 G, 2 sub-steps, tol 1e-6, graphs; G tile autotuned)")
-    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=w.coarse_dtype, substeps=w.substeps,
+    log(f"Parareal solve to {w.tol:g}, G {gdt} ({w.substeps} sub-steps)")
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=gdt, substeps=w.substeps,
                      tol=w.tol, use_graphs=True, autotune=True)
-    log("autotune the one-slice FP64 tile (DoSerial); serial FP64 fine solve")
-    conv.autotune(1)
-    pr.q = serial
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pr.DoSerial()
-    torch.cuda.synchronize()
-    t_serial = time.perf_counter() - t0
-    pr.q = sol
-    # DoParallel (Parareal.h:110-171) step by step, so each iteration logs a line: begin
-    # (initial coarse chain), iterations until the increment test, the last slice's state
-    log("Parareal solve to 1e-6")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pr.begin()
@@ -277,41 +263,71 @@ def cfg4_solve(a, w, conf, conv, init):
         inc = pr.iterate()
         f_s, c_s, _ = pr.iteration_times()
         iters.append({"increment": float(f"{inc:.3e}"), "fine_s": round(f_s, 4), "chain_s": round(c_s, 4)})
-        log(f"iteration {k + 1}: increment {inc:.3e}, fine sweep {f_s:.2f} s, coarse chain {c_s:.3f} s")
+        log(f"  iteration {k + 1}: increment {inc:.3e}, fine sweep {f_s:.2f} s, coarse chain {c_s:.3f} s")
         if inc <= w.tol:
             break
     pr.solution(sol)
     torch.cuda.synchronize()
     t_par = time.perf_counter() - t0
-    K = len(iters)
-    err_l2, err_inf = sp.rel_l2(sol, serial), sp.error_reference(sol, serial)
-    tsG = conf_of(w).timeStepsCoarsePerTimeSlice() * w.substeps
-    dtG = conf.dtCoarse() / w.substeps
     del pr
-    # plain all-FP64 Parareal in the reference's execution structure, from measured step
-    # times: K iterations of S slices x tsF reference-order fine steps, S-1 coarse segments
-    # of tsG FP64 reference-order RK4 sub-steps, plus the initial coarse chain (S segments)
-    log("plain comparator: reference-order step times")
-    f_ms = plain_step_ms(w, dtF)
-    g_ms = plain_step_ms(w, dtG)
-    plain_s = (K * (S * tsF * f_ms + (S - 1) * tsG * g_ms) + S * tsG * g_ms) / 1e3
+    K = len(iters)
+    N, S = w.N, w.slices
     fine_sum = sum(x["fine_s"] for x in iters)
     chain_sum = sum(x["chain_s"] for x in iters)
+    return {"g_dtype": gdt, "wall_s": round(t_par, 3), "iterations": K, "tol": w.tol,
+            "init_coarse_s": round(t_init, 4), "per_iteration": iters,
+            "iteration_over_fine_sweep": round((fine_sum + chain_sum) / fine_sum, 4),
+            "rel_l2_err": float(f"{sp.rel_l2(sol, serial):.3e}"),
+            "rel_linf_err": float(f"{sp.error_reference(sol, serial):.3e}"),
+            "fine_glups_in_solve": round(K * S * conf.timeStepsFinePerTimeSlice() * N ** 3 / fine_sum / 1e9, 3)}
+
+
+def cfg4_solve(a, w, conf, conv, init):
+    """The once-timed runs beside the steps: serial fine, the tolerance-stopped Parareal solve
+    as configured (BF16-storage G), the same with FP64 G (plain all-FP64 Parareal, batched:
+    its iteration count is the plain comparator's) and with FP32 G, and the plain comparator
+    in the reference's execution structure."""
+    import torch
+    N, S = w.N, w.slices
+    tsF, dtF = conf.timeStepsFinePerTimeSlice(), conf.dtFine()
+    sol, serial = conv.field("q"), conv.field("qreference")
+    log("autotune the one-slice FP64 tile (DoSerial); serial FP64 fine solve")
+    conv.autotune(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    conv.DoRK4(init, serial, 0.0, dtF, S * tsF)  # Parareal.h:250-253 (DoSerial)
+    torch.cuda.synchronize()
+    t_serial = time.perf_counter() - t0
+    runs = {}
+    for gdt in ([w.coarse_dtype] + [g for g in a.also_g.split(",") if g and g != w.coarse_dtype]):
+        runs[gdt] = parareal_solve(conv, init, conf, w, sol, serial, gdt)
+    main = runs[w.coarse_dtype]
+    # plain all-FP64 Parareal in the reference's execution structure, from measured step
+    # times: K iterations of S slices x tsF reference-order fine steps, S-1 coarse segments
+    # of tsG FP64 reference-order RK4 sub-steps, plus the initial coarse chain (S segments);
+    # K = the iterations FP64 G takes on this problem (measured above when run)
+    K_plain = runs["f64"]["iterations"] if "f64" in runs else main["iterations"]
+    log("plain comparator: reference-order step times")
+    tsG = conf.timeStepsCoarsePerTimeSlice() * w.substeps
+    dtG = conf.dtCoarse() / w.substeps
+    f_ms = plain_step_ms(w, dtF)
+    g_ms = plain_step_ms(w, dtG)
+    plain_s = (K_plain * (S * tsF * f_ms + (S - 1) * tsG * g_ms) + S * tsG * g_ms) / 1e3
+    t_par = main["wall_s"]
     return {
-        "parareal": {"wall_s": round(t_par, 3), "iterations": K, "tol": w.tol, "init_coarse_s": round(t_init, 4),
-                     "per_iteration": iters,
-                     "iteration_over_fine_sweep": round((fine_sum + chain_sum) / fine_sum, 4),
-                     "rel_l2_err": float(f"{err_l2:.3e}"), "rel_linf_err": float(f"{err_inf:.3e}"),
-                     "fine_glups_in_solve": round(K * S * tsF * N ** 3 / fine_sum / 1e9, 3)},
+        "parareal": main,
+        "parareal_other_g": [runs[g] for g in runs if g != w.coarse_dtype],
         "serial_fine": {"wall_s": round(t_serial, 3), "steps": S * tsF,
                         "glups": round(S * tsF * N ** 3 / t_serial / 1e9, 3)},
         "speedup_vs_serial_fine": round(t_serial / t_par, 4),
         "plain_fp64_parareal_reference_structure": {
-            "wall_s_extrapolated": round(plain_s, 1), "iterations_assumed": K,
+            "wall_s_extrapolated": round(plain_s, 1), "iterations": K_plain,
+            "iterations_from": "the FP64-G solve above" if "f64" in runs else "the configured solve",
             "fine_step_ms": round(f_ms, 4), "coarse_substep_ms": round(g_ms, 4),
             "how": "reference-order sweeps (4 RHS + 3 Euler + 1 RK kernels per step), one slice after another, "
-                   "FP64 G; measured step times x the solve's step counts, same iteration count"},
+                   "FP64 G; measured step times x the solve's step counts"},
         "speedup_vs_plain_fp64_parareal": round(plain_s / t_par, 2),
+        "speedup_vs_plain_fp64_parareal_by_g": {g: round(plain_s / runs[g]["wall_s"], 2) for g in runs},
     }
 
 
@@ -482,6 +498,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the once-timed Parareal, serial and comparator runs")
+    ap.add_argument("--also-g", default="f64,f32",
+                    help="cfg4: G storage types solved beside the configured BF16 one (comma list; '' for none)")
     ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5"], default="cfg4",
                     help="cfg4: 384^3 x 64 slices, BF16 G (default); cfg2: 128^3 x 16 slices, FP64; "
                          "cfg5: the 24-problem uneven batch")
