@@ -50,7 +50,7 @@ extern "C" {
 /* ---- fused-kernel variants ------------------------------------------------------ */
 #define SP_KERNEL_AUTO -1
 #define SP_KERNEL_SCALAR 0 /* one column per lane, any sizes and storage types */
-#define SP_KERNEL_PAIR 1   /* FP64, two x-adjacent columns per lane, 16-byte accesses */
+#define SP_KERNEL_PAIR 1   /* two x-adjacent columns per lane: FP64 (16-byte accesses) or FP32 arithmetic over FP32 / BF16 storage */
 #define SP_KERNEL_QUAD 2   /* FP64, 2x2 columns per lane in stages 1-3 (32x32 tiles, 512 threads) */
 #define SP_KERNEL_SPLIT 3  /* FP64, two launches of a two-stage x-pair kernel through a work field */
 
@@ -116,7 +116,8 @@ int sp_prop_set_path(sp_prop* p, int path);
  * kchunk planes per block (0: whole k extent).  variant SP_KERNEL_AUTO restores the
  * built-in choice.  Compiled shapes: scalar {32x32x1024, 32x24x1024, 24x24x1024,
  * 32x16x512, 16x16x256}, pair {32x32x512, 32x16x512, 16x16x256}, quad {32x32x512}
- * (the built-in choice for FP64 launches of 256 or more blocks), split {32x32x512,
+ * (the built-in choice for FP64 launches of 256 or more blocks; FP32 / BF16 storage takes
+ * the pair kernel in its place), split {32x32x512,
  * 32x16x512, 32x16x256, 16x16x256}.  The split variant (FP64 fields, even ni) runs each
  * step as two launches through a work field the propagator allocates (8 bytes per point
  * of the batch) when the tile is chosen; its results equal the other FP64 variants' bit
@@ -126,9 +127,10 @@ int sp_prop_set_tile(sp_prop* p, int variant, int tx, int ty, int nthreads, int 
 /* The fused-kernel tile sp_do_rk4 (nslices = 1) or sp_do_rk4_batched (nslices > 1) will
  * use for FP64 compute (f64 = 1) or reduced-precision storage (f64 = 0): the one set with
  * sp_prop_set_tile, else the built-in choice for this size.  Output pointers may be NULL.
- * Fields the pair and quad kernels cannot take (odd ni, odd strides, origins not 16-byte
- * aligned) run the scalar 32x16x512 kernel instead; the report covers odd ni (no
- * counterpart in the reference). */
+ * Fields the pair, quad and split kernels cannot take (odd ni, odd strides, origins not
+ * aligned to two elements; quad and split: storage other than FP64) run the scalar
+ * 32x16x512 kernel instead; the report covers odd ni and f64 = 0 (no counterpart in the
+ * reference). */
 int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* tx, int* ty, int* nthreads,
                      int* kchunk);
 /* Autotuner (no counterpart in the reference, whose CUDA block size is fixed at 32x4):

@@ -575,8 +575,9 @@ int sp_prop_get_tile(const sp_prop* p, int nslices, int f64, int* variant, int* 
     SP_CHECK(p, SP_ERR_INVALID, "sp_prop_get_tile: null propagator");
     SP_CHECK(nslices >= 1, SP_ERR_INVALID, "sp_prop_get_tile: nslices < 1");
     FusedTile t = pick_tile(p, nslices, f64 ? 1 : 0);
-    // the 16-byte kernels need FP64 and an even x extent (as every layout then has even strides)
-    if ((!f64 || p->ni % 2 != 0) && t.variant != SP_KERNEL_SCALAR)
+    // the pair kernel needs an even x extent (as every layout then has even strides); the
+    // 2x2 and split kernels also FP64
+    if (t.variant != SP_KERNEL_SCALAR && (p->ni % 2 != 0 || (!f64 && t.variant != SP_KERNEL_PAIR)))
         t = FusedTile{32, 16, 512, t.kchunk, SP_KERNEL_SCALAR};
     if (variant) *variant = t.variant;
     if (tx) *tx = t.tx;

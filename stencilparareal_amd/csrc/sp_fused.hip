@@ -443,6 +443,56 @@ __device__ __forceinline__ void st2_stream(__amdgpu_buffer_rsrc_t r, int vo, int
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, vo, so, 2);
 }
 
+// Two x-adjacent values of compute type T (d2 for FP64), and their global access for
+// storage type S: 2 * sizeof(S) bytes per lane (FP64 16, FP32 8, BF16 4), converted to T.
+template <typename T>
+struct V2 {
+    typedef T type __attribute__((ext_vector_type(2)));
+};
+template <typename S, typename T>
+struct PairIO;
+template <>
+struct PairIO<double, double> {
+    __device__ __forceinline__ static d2 ld(__amdgpu_buffer_rsrc_t r, int vo, int so) { return ld2(r, vo, so); }
+    __device__ __forceinline__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, d2 x) { st2(r, vo, so, x); }
+};
+template <>
+struct PairIO<float, float> {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __device__ __forceinline__ static f2 ld(__amdgpu_buffer_rsrc_t r, int vo, int so)
+    {
+        return __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    }
+    __device__ __forceinline__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, f2 x)
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, vo, so, 0);
+    }
+};
+template <>
+struct PairIO<double, float> {  // FP64 input of a reduced-precision step: rounded to FP32 on load
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    __device__ __forceinline__ static f2 ld(__amdgpu_buffer_rsrc_t r, int vo, int so)
+    {
+        const d2 v = ld2(r, vo, so);
+        return f2{(float)v.x, (float)v.y};
+    }
+};
+template <>
+struct PairIO<bf16, float> {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    __device__ __forceinline__ static f2 ld(__amdgpu_buffer_rsrc_t r, int vo, int so)
+    {
+        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
+        return f2{bf16_to_f32((uint16_t)(v & 0xffffu)), bf16_to_f32((uint16_t)(v >> 16))};
+    }
+    __device__ __forceinline__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, f2 x)
+    {
+        const unsigned v = (unsigned)f32_to_bf16(x.x) | ((unsigned)f32_to_bf16(x.y) << 16);
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, vo, so, 0);
+    }
+};
+
 // Lane-slot layout of stage S's output pairs (WP pairs x H rows).  Slot g maps to pair
 // (ax / 2, by), row-major with VW slots per row.  With VW equal to the input plane's row
 // stride in pairs (WI / 2), consecutive slots read consecutive 16-byte LDS chunks across
@@ -485,30 +535,31 @@ struct PairMap {
     __device__ static bool live(int g) { return g < SLOTS && g % VW < WP; }
 };
 
-template <int TX, int TY, int S, int NT, int M, int R, int WI = Geo<TX, TY, S>::WI>
-__device__ __forceinline__ void pair_step(const double* __restrict__ P, const StarW<double>& w, double (&acc)[M][2][4],
-                                          double (&done)[M][2], d2 (&cen)[M])
+template <int TX, int TY, int S, int NT, int M, int R, int WI = Geo<TX, TY, S>::WI, typename T>
+__device__ __forceinline__ void pair_step(const T* __restrict__ P, const StarW<T>& w, T (&acc)[M][2][4],
+                                          T (&done)[M][2], typename V2<T>::type (&cen)[M])
 {
+    using v2 = typename V2<T>::type;
     constexpr int sm2 = (R + 2) & 3, sm1 = (R + 3) & 3, s0 = R & 3, sp1 = (R + 1) & 3;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const double* c = P + PairMap<TX, TY, S, NT>::template centre<WI>((int)threadIdx.x + m * NT);
+        const T* c = P + PairMap<TX, TY, S, NT>::template centre<WI>((int)threadIdx.x + m * NT);
         // x group first (centres and x neighbours), then the y group: at most 4 pairs of
         // loaded values are live at a time
-        const d2 C = *(const d2*)c, Lx = *(const d2*)(c - 2), Rx = *(const d2*)(c + 2);
+        const v2 C = *(const v2*)c, Lx = *(const v2*)(c - 2), Rx = *(const v2*)(c + 2);
         cen[m] = C;
-        double ax[2];
+        T ax[2];
         {
-            const double u[2] = {C.x, C.y};
+            const T u[2] = {C.x, C.y};
             // x neighbours of the two columns: (Lx.x, Lx.y, C.y, Rx.x) and (Lx.y, C.x, Rx.x, Rx.y)
-            const double xm2[2] = {Lx.x, Lx.y}, xm1[2] = {Lx.y, C.x}, xp1[2] = {C.y, Rx.x}, xp2[2] = {Rx.x, Rx.y};
+            const T xm2[2] = {Lx.x, Lx.y}, xm1[2] = {Lx.y, C.x}, xp1[2] = {C.y, Rx.x}, xp2[2] = {Rx.x, Rx.y};
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 done[m][e] = fmaT(w.zp2, u[e], acc[m][e][sm2]);
                 acc[m][e][sm2] = w.zm2 * u[e];
                 acc[m][e][sm1] = fmaT(w.zp1, u[e], acc[m][e][sm1]);
                 acc[m][e][sp1] = fmaT(w.zm1, u[e], acc[m][e][sp1]);
-                double a = fmaT(w.c, u[e], acc[m][e][s0]);
+                T a = fmaT(w.c, u[e], acc[m][e][s0]);
                 a = fmaT(w.xm2, xm2[e], a);
                 a = fmaT(w.xm1, xm1[e], a);
                 a = fmaT(w.xp1, xp1[e], a);
@@ -517,13 +568,13 @@ __device__ __forceinline__ void pair_step(const double* __restrict__ P, const St
         }
         __builtin_amdgcn_sched_barrier(0);
         {
-            const d2 Ym2 = *(const d2*)(c - 2 * WI), Ym1 = *(const d2*)(c - WI);
-            const d2 Yp1 = *(const d2*)(c + WI), Yp2 = *(const d2*)(c + 2 * WI);
-            const double ym2[2] = {Ym2.x, Ym2.y}, ym1[2] = {Ym1.x, Ym1.y}, yp1[2] = {Yp1.x, Yp1.y},
-                         yp2[2] = {Yp2.x, Yp2.y};
+            const v2 Ym2 = *(const v2*)(c - 2 * WI), Ym1 = *(const v2*)(c - WI);
+            const v2 Yp1 = *(const v2*)(c + WI), Yp2 = *(const v2*)(c + 2 * WI);
+            const T ym2[2] = {Ym2.x, Ym2.y}, ym1[2] = {Ym1.x, Ym1.y}, yp1[2] = {Yp1.x, Yp1.y},
+                    yp2[2] = {Yp2.x, Yp2.y};
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                double a = w.ym2 * ym2[e];
+                T a = w.ym2 * ym2[e];
                 a = fmaT(w.ym1, ym1[e], a);
                 a = fmaT(w.yp1, yp1[e], a);
                 a = fmaT(w.yp2, yp2[e], a);
@@ -534,7 +585,8 @@ __device__ __forceinline__ void pair_step(const double* __restrict__ P, const St
     }
 }
 
-template <int TX, int TY, int S, int NT, int M>
+// Byte offsets (element size ES) of stage S's output pairs in a field (periodic images).
+template <int TX, int TY, int S, int NT, int M, int ES = 8>
 __device__ __forceinline__ void pair_offsets(int (&q)[M], int x0, int y0, int ni, int nj, int sj)
 {
     using G = Geo<TX, TY, S>;
@@ -542,15 +594,15 @@ __device__ __forceinline__ void pair_offsets(int (&q)[M], int x0, int y0, int ni
     for (int m = 0; m < M; ++m) {
         int ax, by;
         PairMap<TX, TY, S, NT>::at((int)threadIdx.x + m * NT, ax, by);
-        q[m] = (pmod(x0 - G::HO + ax, ni) + pmod(y0 - G::HO + by, nj) * sj) * 8;
+        q[m] = (pmod(x0 - G::HO + ax, ni) + pmod(y0 - G::HO + by, nj) * sj) * ES;
     }
 }
 
-template <int M>
-__device__ __forceinline__ void pair_qload(d2 (&qv)[M], __amdgpu_buffer_rsrc_t r, int zo, const int (&q)[M])
+template <typename SI, int M, typename T>
+__device__ __forceinline__ void pair_qload(T (&qv)[M], __amdgpu_buffer_rsrc_t r, int zo, const int (&q)[M])
 {
 #pragma unroll
-    for (int m = 0; m < M; ++m) qv[m] = ld2(r, q[m], zo);
+    for (int m = 0; m < M; ++m) qv[m] = PairIO<SI, decltype(qv[m].x + 0)>::ld(r, q[m], zo);
 }
 
 // Index of pair g of stage S's output region inside a plane of region stage S+d
@@ -571,7 +623,10 @@ __device__ __forceinline__ void pair_qidx(int (&same)[M], int (&next)[M])
     }
 }
 
-// FP64 RK4 step, two x-adjacent columns per lane.  The stage updates v_s = q + f_s L v_{s-1}
+// RK4 step, two x-adjacent columns per lane: FP64 (SI = SO = T = double), or FP32 arithmetic
+// over FP32 / BF16 storage (T = float; SI = double for a step from an FP64 input).  The
+// arithmetic is the scalar kernel's for the same types, so both give the same bits.  The
+// stage updates v_s = q + f_s L v_{s-1}
 // need q at each stage's output plane, which lags the q plane being loaded by 2, 5, 8 and
 // 11 planes.  Reloading it from global memory at those lags keeps a 12-plane window per
 // block live in L2, more than an XCD's L2 holds for its resident blocks.  Instead:
@@ -579,9 +634,11 @@ __device__ __forceinline__ void pair_qidx(int (&same)[M], int (&next)[M])
 //     ring), stage 2 reloads q at lag 5 (a window the L2 holds), and writes it into an
 //     LDS plane Q3 beside its output; stage 3 reads q from Q3 with its input plane, keeps
 //     it 2 planes in registers and forwards it through Q4 to stage 4 the same way.
-template <int TX, int TY, int NT>
+template <typename SI, typename SO, typename T, int TX, int TY, int NT>
 __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 {
+    using v2 = typename V2<T>::type;
+    constexpr int ESI = sizeof(SI), ESO = sizeof(SO);
     using G1 = Geo<TX, TY, 1>;
     using G2 = Geo<TX, TY, 2>;
     using G3 = Geo<TX, TY, 3>;
@@ -594,12 +651,12 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
     using PM4 = PairMap<TX, TY, 4, NT>;
     constexpr int M1 = PM1::M, M2 = PM2::M, M3 = PM3::M, M4 = PM4::M;
 
-    __shared__ __attribute__((aligned(16))) double P0[2][G1::AI];
-    __shared__ __attribute__((aligned(16))) double P1[2][G2::AI];
-    __shared__ __attribute__((aligned(16))) double P2[2][G3::AI];
-    __shared__ __attribute__((aligned(16))) double P3[2][G4::AI];
-    __shared__ __attribute__((aligned(16))) double Q3[2][G3::AO + 2];  // q on stage 3's region
-    __shared__ __attribute__((aligned(16))) double Q4[2][G4::AO + 2];  // q on stage 4's region
+    __shared__ __attribute__((aligned(16))) T P0[2][G1::AI];
+    __shared__ __attribute__((aligned(16))) T P1[2][G2::AI];
+    __shared__ __attribute__((aligned(16))) T P2[2][G3::AI];
+    __shared__ __attribute__((aligned(16))) T P3[2][G4::AI];
+    __shared__ __attribute__((aligned(16))) T Q3[2][G3::AO + 2];  // q on stage 3's region
+    __shared__ __attribute__((aligned(16))) T Q4[2][G4::AO + 2];  // q on stage 4's region
 
     const int tid = threadIdx.x;
     const int ni = a.ni, nj = a.nj, nk = a.nk;
@@ -609,26 +666,26 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
     const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
     const int z0 = chunk * a.kchunk;
     const int zend = min(z0 + a.kchunk, nk);
-    const __amdgpu_buffer_rsrc_t rin = make_rsrc((const double*)a.in.p + slice * a.in.slice, a.in_bytes);
-    const __amdgpu_buffer_rsrc_t rout = make_rsrc((double*)a.out.p + slice * a.out.slice, a.out_bytes);
-    const int iskb = (int)a.in.sk * 8, oskb = (int)a.out.sk * 8;
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc((const SI*)a.in.p + slice * a.in.slice, a.in_bytes);
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc((SO*)a.out.p + slice * a.out.slice, a.out_bytes);
+    const int iskb = (int)a.in.sk * ESI, oskb = (int)a.out.sk * ESO;
     const int isj = (int)a.in.sj, osj = (int)a.out.sj;
-    const StarW<double> w = a.cdev ? load_w<double>(a.cdev[slice]) : load_w<double>(a.c);
+    const StarW<T> w = a.cdev ? load_w<T>(a.cdev[slice]) : load_w<T>(a.c);
 
     int off0[M0];
 #pragma unroll
     for (int m = 0; m < M0; ++m) {
         const int g = min(tid + m * NT, A0P - 1);
         const int ax = 2 * (g % (G1::WI / 2)), by = g / (G1::WI / 2);
-        off0[m] = (pmod(x0 - 8 + ax, ni) + pmod(y0 - 8 + by, nj) * isj) * 8;
+        off0[m] = (pmod(x0 - 8 + ax, ni) + pmod(y0 - 8 + by, nj) * isj) * ESI;
     }
     int q2[M2], o4[M4];
-    pair_offsets<TX, TY, 2, NT, M2>(q2, x0, y0, ni, nj, isj);
+    pair_offsets<TX, TY, 2, NT, M2, ESI>(q2, x0, y0, ni, nj, isj);
 #pragma unroll
     for (int m = 0; m < M4; ++m) {
         int ax, by;
         PM4::at(tid + m * NT, ax, by);
-        o4[m] = (pmod(x0 + ax, ni) + pmod(y0 + by, nj) * osj) * 8;  // periodic image for partial tiles
+        o4[m] = (pmod(x0 + ax, ni) + pmod(y0 + by, nj) * osj) * ESO;  // periodic image for partial tiles
     }
     int s2[M2], w3[M2], r3x[M3], w4[M3], r4x[M4], n4[M4];
     // slots past a stage's last pair and the padding slots of padded rows repeat a pair:
@@ -651,30 +708,30 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
     pair_qidx<TX, TY, 3, NT, M3>(r3x, w4);
     pair_qidx<TX, TY, 4, NT, M4>(r4x, n4);
 
-    double acc1[M1][2][4], acc2[M2][2][4], acc3[M3][2][4], acc4[M4][2][4];
-    double d1[M1][2], d2v[M2][2], d3[M3][2], d4[M4][2];
-    d2 qv2[M2];
-    d2 c1[M1], c2[M2], c3[M3], c4[M4];                   // centres read this plane
+    T acc1[M1][2][4], acc2[M2][2][4], acc3[M3][2][4], acc4[M4][2][4];
+    T d1[M1][2], d2v[M2][2], d3[M3][2], d4[M4][2];
+    v2 qv2[M2];
+    v2 c1[M1], c2[M2], c3[M3], c4[M4];                   // centres read this plane
     // q one and two planes back, in two-slot rings indexed by the plane's parity (R & 1 in
     // the unrolled loop): slot B holds the value stored two planes ago
-    d2 r1[2][M1], r3[2][M3], r4[2][M4];
+    v2 r1[2][M1], r3[2][M3], r4[2][M4];
 #pragma unroll
     for (int m = 0; m < M1; ++m) {
-        r1[0][m] = r1[1][m] = d2{0., 0.};
-        for (int e = 0; e < 2; ++e) acc1[m][e][0] = acc1[m][e][1] = acc1[m][e][2] = acc1[m][e][3] = 0.;
+        r1[0][m] = r1[1][m] = v2{T(0), T(0)};
+        for (int e = 0; e < 2; ++e) acc1[m][e][0] = acc1[m][e][1] = acc1[m][e][2] = acc1[m][e][3] = T(0);
     }
 #pragma unroll
     for (int m = 0; m < M2; ++m)
-        for (int e = 0; e < 2; ++e) acc2[m][e][0] = acc2[m][e][1] = acc2[m][e][2] = acc2[m][e][3] = 0.;
+        for (int e = 0; e < 2; ++e) acc2[m][e][0] = acc2[m][e][1] = acc2[m][e][2] = acc2[m][e][3] = T(0);
 #pragma unroll
     for (int m = 0; m < M3; ++m) {
-        r3[0][m] = r3[1][m] = d2{0., 0.};
-        for (int e = 0; e < 2; ++e) acc3[m][e][0] = acc3[m][e][1] = acc3[m][e][2] = acc3[m][e][3] = 0.;
+        r3[0][m] = r3[1][m] = v2{T(0), T(0)};
+        for (int e = 0; e < 2; ++e) acc3[m][e][0] = acc3[m][e][1] = acc3[m][e][2] = acc3[m][e][3] = T(0);
     }
 #pragma unroll
     for (int m = 0; m < M4; ++m) {
-        r4[0][m] = r4[1][m] = d2{0., 0.};
-        for (int e = 0; e < 2; ++e) acc4[m][e][0] = acc4[m][e][1] = acc4[m][e][2] = acc4[m][e][3] = 0.;
+        r4[0][m] = r4[1][m] = v2{T(0), T(0)};
+        for (int e = 0; e < 2; ++e) acc4[m][e][0] = acc4[m][e][1] = acc4[m][e][2] = acc4[m][e][3] = T(0);
     }
 
     // Same skewed schedule as k_rk4_fused: iteration t, stage 1: q t -> v1 t-2; stage 2:
@@ -687,14 +744,14 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         const int zo = wrapz(tq0, nk) * iskb;
 #pragma unroll
         for (int m = 0; m < M0; ++m)
-            if (wr0(m)) *(d2*)&P0[tq0 & 1][2 * min(tid + m * NT, A0P - 1)] = ld2(rin, off0[m], zo);
-        pair_qload<M2>(qv2, rin, wrapz(z0 - 8, nk) * iskb, q2);
+            if (wr0(m)) *(v2*)&P0[tq0 & 1][2 * min(tid + m * NT, A0P - 1)] = PairIO<SI, T>::ld(rin, off0[m], zo);
+        pair_qload<SI>(qv2, rin, wrapz(z0 - 8, nk) * iskb, q2);
     }
-    d2 pre[M0];
+    v2 pre[M0];
     if (tq0 + 1 < tq1) {
         const int zo = wrapz(tq0 + 1, nk) * iskb;
 #pragma unroll
-        for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+        for (int m = 0; m < M0; ++m) pre[m] = PairIO<SI, T>::ld(rin, off0[m], zo);
     }
     __syncthreads();
 
@@ -706,11 +763,11 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
         if (t + 1 < tq1) {
 #pragma unroll
             for (int m = 0; m < M0; ++m)
-                if (wr0(m)) *(d2*)&P0[prv][2 * min(tid + m * NT, A0P - 1)] = pre[m];
+                if (wr0(m)) *(v2*)&P0[prv][2 * min(tid + m * NT, A0P - 1)] = pre[m];
             if (t + 2 < tq1) {
                 const int zo = wrapz(t + 2, nk) * iskb;
 #pragma unroll
-                for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+                for (int m = 0; m < M0; ++m) pre[m] = PairIO<SI, T>::ld(rin, off0[m], zo);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -720,8 +777,8 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 #pragma unroll
             for (int m = 0; m < M1; ++m) {
                 if (!MASK || lv1[m])
-                    *(d2*)&P1[cur][PM1::out(tid + m * NT)] =
-                        d2{fmaT(w.f1, d1[m][0], r1[B][m].x), fmaT(w.f1, d1[m][1], r1[B][m].y)};
+                    *(v2*)&P1[cur][PM1::out(tid + m * NT)] =
+                        v2{fmaT(w.f1, d1[m][0], r1[B][m].x), fmaT(w.f1, d1[m][1], r1[B][m].y)};
                 r1[B][m] = c1[m];
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -731,23 +788,23 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 #pragma unroll
             for (int m = 0; m < M2; ++m) {
                 if (!MASK || lv2[m]) {
-                    *(d2*)&P2[cur][PM2::out(tid + m * NT)] =
-                        d2{fmaT(w.f2, d2v[m][0], qv2[m].x), fmaT(w.f2, d2v[m][1], qv2[m].y)};
-                    *(d2*)&Q3[cur][w3[m]] = qv2[m];  // q(t-5) for stage 3
+                    *(v2*)&P2[cur][PM2::out(tid + m * NT)] =
+                        v2{fmaT(w.f2, d2v[m][0], qv2[m].x), fmaT(w.f2, d2v[m][1], qv2[m].y)};
+                    *(v2*)&Q3[cur][w3[m]] = qv2[m];  // q(t-5) for stage 3
                 }
             }
-            pair_qload<M2>(qv2, rin, wrapz(t - 4, nk) * iskb, q2);
+            pair_qload<SI>(qv2, rin, wrapz(t - 4, nk) * iskb, q2);
             __builtin_amdgcn_sched_barrier(0);
         }
         if (t >= tq0 + 10 && t < tq1 + 2) {
             pair_step<TX, TY, 3, NT, M3, (R + 2) & 3>(P2[prv], w, acc3, d3, c3);
 #pragma unroll
             for (int m = 0; m < M3; ++m) {
-                const d2 qn = *(const d2*)&Q3[prv][r3x[m]];  // q(t-6), written by stage 2 last plane
+                const v2 qn = *(const v2*)&Q3[prv][r3x[m]];  // q(t-6), written by stage 2 last plane
                 if (!MASK || lv3[m]) {
-                    *(d2*)&P3[cur][PM3::out(tid + m * NT)] =
-                        d2{fmaT(w.f3, d3[m][0], r3[B][m].x), fmaT(w.f3, d3[m][1], r3[B][m].y)};
-                    *(d2*)&Q4[cur][w4[m]] = r3[B][m];  // q(t-8) for stage 4
+                    *(v2*)&P3[cur][PM3::out(tid + m * NT)] =
+                        v2{fmaT(w.f3, d3[m][0], r3[B][m].x), fmaT(w.f3, d3[m][1], r3[B][m].y)};
+                    *(v2*)&Q4[cur][w4[m]] = r3[B][m];  // q(t-8) for stage 4
                 }
                 r3[B][m] = qn;
             }
@@ -761,10 +818,11 @@ __device__ __forceinline__ void rk4_pair_body(const FusedArgs& a, const int bid)
 #pragma unroll
                 for (int m = 0; m < M4; ++m)
                     if (!MASK || lv4[m])
-                        st2(rout, o4[m], zo, d2{fmaT(w.f4, d4[m][0], r4[B][m].x), fmaT(w.f4, d4[m][1], r4[B][m].y)});
+                        PairIO<SO, T>::st(rout, o4[m], zo,
+                                          v2{fmaT(w.f4, d4[m][0], r4[B][m].x), fmaT(w.f4, d4[m][1], r4[B][m].y)});
             }
 #pragma unroll
-            for (int m = 0; m < M4; ++m) r4[B][m] = *(const d2*)&Q4[prv][r4x[m]];  // q(t-9)
+            for (int m = 0; m < M4; ++m) r4[B][m] = *(const v2*)&Q4[prv][r4x[m]];  // q(t-9)
             __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
@@ -1303,10 +1361,10 @@ __global__ void __launch_bounds__(NT) k_rk4_fused(FusedArgs a)
     rk4_scalar_body<SI, SO, T, TX, TY, NT>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
-template <int TX, int TY, int NT>
+template <typename SI, typename SO, typename T, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_rk4_fused_pair(FusedArgs a)
 {
-    rk4_pair_body<TX, TY, NT>(a, xcd_block(blockIdx.x, gridDim.x));
+    rk4_pair_body<SI, SO, T, TX, TY, NT>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 __global__ void __launch_bounds__(QuadCfg::NT) k_rk4_fused_quad(FusedArgs a)
@@ -1349,7 +1407,7 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_pair_grouped(GroupArgs g)
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int e = group_entry(g, bid);
     ConstArgs* rec = (ConstArgs*)g.args + e;
-    rk4_pair_body<TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
+    rk4_pair_body<double, double, double, TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
 }
 
 __global__ void __launch_bounds__(QuadCfg::NT) k_rk4_fused_quad_grouped(GroupArgs g)
@@ -1438,16 +1496,17 @@ int launch_typed(hipStream_t s, const FusedStep& st, const FusedTile& tile, View
     return fail(SP_ERR_UNSUPPORTED, "fused RK4: unsupported tile shape");
 }
 
+template <typename SI, typename SO, typename T>
 int launch_pair(hipStream_t s, const FusedStep& st, const FusedTile& tile, View in, View out)
 {
     FusedArgs a;
-    SP_TRY(fill_args(a, st, tile, in, out, 8, 8));
+    SP_TRY(fill_args(a, st, tile, in, out, (int)sizeof(SI), (int)sizeof(SO)));
     const dim3 grid(nblocks_of(a, st.nslices));
-#define SP_PAIR_CASE(TX_, TY_, NT_)                                                      \
-    if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                            \
-        hipLaunchKernelGGL((k_rk4_fused_pair<TX_, TY_, NT_>), grid, dim3(NT_), 0, s, a); \
-        SP_HIP(hipGetLastError());                                                       \
-        return SP_OK;                                                                    \
+#define SP_PAIR_CASE(TX_, TY_, NT_)                                                                  \
+    if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                                        \
+        hipLaunchKernelGGL((k_rk4_fused_pair<SI, SO, T, TX_, TY_, NT_>), grid, dim3(NT_), 0, s, a); \
+        SP_HIP(hipGetLastError());                                                                   \
+        return SP_OK;                                                                                \
     }
     SP_PAIR_CASE(32, 32, 512)
     SP_PAIR_CASE(32, 16, 512)
@@ -1501,12 +1560,13 @@ int launch_split(hipStream_t s, const FusedStep& st, const FusedTile& tile, View
     return fail(SP_ERR_UNSUPPORTED, "fused RK4 (split): unsupported tile shape");
 }
 
-// 16-byte pair access needs even x extent, even strides and 16-byte aligned origins.
-bool pair_ok(const FusedStep& st, View in, View out)
+// Pair access (two elements per lane) needs even x extent, even strides and origins aligned
+// to two elements of each field's storage type (FP64: 16 bytes).
+bool pair_ok(const FusedStep& st, View in, View out, int isz = 8, int osz = 8)
 {
     const auto even = [](long long v) { return (v & 1) == 0; };
     return even(st.ni) && even(in.sj) && even(in.sk) && even(out.sj) && even(out.sk) && even(in.slice) &&
-           even(out.slice) && ((uintptr_t)in.p & 15) == 0 && ((uintptr_t)out.p & 15) == 0;
+           even(out.slice) && ((uintptr_t)in.p % (2 * isz)) == 0 && ((uintptr_t)out.p % (2 * osz)) == 0;
 }
 
 template <typename SI, typename SO, typename T>
@@ -1607,22 +1667,22 @@ FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64)
     // Measured on MI355X (FP64; 16 x 128^3, 8 x 160^3, 24 x 96^3, 1 x 128^3): the 2x2
     // kernel (32x32 tiles of 512 threads) is fastest once the tiles alone give one block
     // per CU; below that, x-pair 32x16 tiles chunked in k fill the chip better.  FP32 / BF16
-    // storage runs the scalar kernel.
+    // storage runs the x-pair kernel in the 2x2 kernel's place (16 x 128^3: 218 us against
+    // 283 us for the scalar kernel's best tile; 1 x 384^3 in 64-plane chunks: 479 against
+    // 574 us).  Odd x extents fall back to the scalar kernel at launch.
     const long long want = 256;  // one block per CU
-    FusedTile t{32, 32, 512, 0, compute_f64 ? SP_KERNEL_QUAD : SP_KERNEL_SCALAR};
-    if (!compute_f64) t.nt = 1024;
+    FusedTile t{32, 32, 512, 0, compute_f64 ? SP_KERNEL_QUAD : SP_KERNEL_PAIR};
     long long blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
-    // Long k columns (one 384^3 slice: 144 tiles): 64-plane chunks of 2x2 tiles give three
-    // or more blocks per CU at a 19-plane pipeline fill (measured: 0.55 ms per 384^3 step,
-    // against 1.07 ms for whole-k 32x16 tiles, whose 288 blocks leave a ragged second wave).
-    if (compute_f64 && blocks < want && blocks * (nk / 64) >= 3 * want) {
+    // Long k columns (one 384^3 slice: 144 tiles): 64-plane chunks of 32x32 tiles give three
+    // or more blocks per CU at a 19-plane pipeline fill (measured, FP64: 0.55 ms per 384^3
+    // step, against 1.07 ms for whole-k 32x16 tiles, whose 288 blocks leave a ragged second wave).
+    if (blocks < want && blocks * (nk / 64) >= 3 * want) {
         t.kchunk = 64;
         return t;
     }
     if (blocks < want) {
         t.ty = 16;
-        t.nt = 512;
-        if (compute_f64) t.variant = SP_KERNEL_PAIR;
+        t.variant = SP_KERNEL_PAIR;
         blocks = (long long)((ni + t.tx - 1) / t.tx) * ((nj + t.ty - 1) / t.ty) * nslices;
     }
     int chunks = (int)((want + blocks - 1) / blocks);
@@ -1639,6 +1699,8 @@ void fused_candidates(int ni, int nj, int nk, int nslices, int compute_f64, std:
     static const int pair[][3] = {{32, 32, 512}, {32, 16, 512}, {16, 16, 256}};
     static const int split[][3] = {{32, 32, 512}, {32, 16, 512}, {32, 16, 256}, {16, 16, 256}};
     std::vector<FusedTile> shapes;
+    if (!compute_f64 && ni % 2 == 0)
+        for (const auto& sh : pair) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_PAIR});
     if (compute_f64 && ni % 2 == 0) {
         shapes.push_back(FusedTile{QuadCfg::TX, QuadCfg::TY, QuadCfg::NT, 0, SP_KERNEL_QUAD});
         for (const auto& sh : pair) shapes.push_back(FusedTile{sh[0], sh[1], sh[2], 0, SP_KERNEL_PAIR});
@@ -1675,7 +1737,15 @@ int launch_rk4_fused(hipStream_t s, const FusedStep& st, const FusedTile& tile_i
         return fail(SP_ERR_UNSUPPORTED, "fused RK4: ksize " + std::to_string(st.nk) + " < " +
                                             std::to_string(kFusedMinK) + " (use the staged path)");
     if (f64 && tile.variant == SP_KERNEL_QUAD && pair_ok(st, in, out)) return launch_quad(s, st, tile, in, out);
-    if (f64 && tile.variant == SP_KERNEL_PAIR && pair_ok(st, in, out)) return launch_pair(s, st, tile, in, out);
+    if (f64 && tile.variant == SP_KERNEL_PAIR && pair_ok(st, in, out))
+        return launch_pair<double, double, double>(s, st, tile, in, out);
+    if (!f64 && tile.variant == SP_KERNEL_PAIR &&
+        pair_ok(st, in, out, (int)dtype_size(in_dtype), (int)dtype_size(out_dtype))) {
+        if (in_dtype == SP_F64 && out_dtype == SP_F32) return launch_pair<double, float, float>(s, st, tile, in, out);
+        if (in_dtype == SP_F32 && out_dtype == SP_F32) return launch_pair<float, float, float>(s, st, tile, in, out);
+        if (in_dtype == SP_F64 && out_dtype == SP_BF16) return launch_pair<double, bf16, float>(s, st, tile, in, out);
+        if (in_dtype == SP_BF16 && out_dtype == SP_BF16) return launch_pair<bf16, bf16, float>(s, st, tile, in, out);
+    }
     if (f64 && tile.variant == SP_KERNEL_SPLIT && pair_ok(st, in, out)) return launch_split(s, st, tile, in, out);
     if (tile.variant == SP_KERNEL_PAIR || tile.variant == SP_KERNEL_QUAD || tile.variant == SP_KERNEL_SPLIT) {
         // odd sizes, unaligned views or reduced-precision storage: the scalar kernel with

@@ -240,11 +240,15 @@ def test_fused_rk4_long_run(sp):
 # Each step stores the state once: relative rounding <= 2^-24 (f32) or 2^-9 (bf16) per
 # store, so after `steps` stores the error is bounded by steps * that (plus the f32
 # arithmetic); the bounds below are that worst case for 20 steps.
+@pytest.mark.parametrize("variant", [None, "pair"])
 @pytest.mark.parametrize("dtype,tol", [("f32", 20 * 2.0 ** -24 * 4), ("bf16", 20 * 2.0 ** -9)])
-def test_reduced_precision_rk4_against_f64_oracle(sp, dtype, tol):
+def test_reduced_precision_rk4_against_f64_oracle(sp, dtype, tol, variant):
     N, steps = 32, 20
     q0 = smooth_field(N, 11)
     conv = sp.Convection(N, N, N, 1. / N, 5e-3, 0.0, *C3)
+    if variant:
+        conv.set_tile(32, 16, 512, 16, variant=variant)
+        assert conv.get_tile(1, False)["variant"] == variant
     a, b = conv.field(), conv.field(dtype=dtype)
     a.set(q0)
     conv.DoRK4(a, b, 0.0, 1e-3, steps)
@@ -253,6 +257,26 @@ def test_reduced_precision_rk4_against_f64_oracle(sp, dtype, tol):
     print(f"{dtype}: rel err {err:.3e} (bound {tol:.3e})")
     assert err <= tol
     assert err > 1e-14  # it really ran in reduced precision
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_reduced_precision_pair_and_scalar_kernels_agree_bitwise(sp, dtype):
+    # FP32 arithmetic over FP32 / BF16 storage: the x-pair kernel (8- / 4-byte accesses) and
+    # the scalar kernel evaluate every point with the same operations: the first step from
+    # the FP64 field, then steps in the storage type, partial tiles, k chunks
+    ni, nj, nk = 40, 26, 24
+    q0 = rand_field((nk, nj, ni), 12)
+    outs = []
+    for variant, tile in (("scalar", (32, 16, 512)), ("pair", (32, 16, 512)), ("pair", (32, 32, 512)),
+                          ("pair", (16, 16, 256))):
+        conv = sp.Convection(ni, nj, nk, 1. / 32, 0.01, 30.0, *C3)
+        conv.set_tile(*tile, kchunk=8, variant=variant)
+        a, b = conv.field(), conv.field(dtype=dtype)
+        a.set(q0)
+        conv.DoRK4(a, b, 0.1, 2e-4, 4)
+        outs.append(b.get())
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
 
 
 def test_periodic_fill_bitwise(sp):
