@@ -63,13 +63,15 @@ def conf_of(w, **kw):
 class Batch2:
     """Two device arrays of S slices (ping-pong) and the batched fused fine step over them."""
 
-    def __init__(self, conv, init, S):
+    def __init__(self, conv, init, S, dtype="f64"):
         import torch
-        self.conv, self.S, self.lay = conv, S, init.layout
+        self.conv, self.S = conv, S
+        self.lay = init.layout if dtype == "f64" else conv.field("layout_only", dtype=dtype).layout
         self.n = init.layout.ni * init.layout.nj * init.layout.nk
-        self.a = torch.empty(S * self.n, dtype=torch.float64, device="cuda")
+        tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[dtype]
+        self.a = torch.empty(S * self.n, dtype=tdt, device="cuda")
         self.b = torch.empty_like(self.a)
-        src = init.interior().reshape(-1)
+        src = init.interior().reshape(-1).to(tdt)
         for s in range(S):
             self.a[s * self.n:(s + 1) * self.n].copy_(src)
         conv.ctx.sync_stream()
@@ -93,7 +95,7 @@ class Batch2:
         return torch
 
     def copy_gbs(self, reps=10):
-        """Bandwidth of a plain device copy of the same bytes (read S*n doubles, write as many)."""
+        """Bandwidth of a plain device copy of the same bytes (read S*n elements, write as many)."""
         import torch
         self.b.copy_(self.a)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -102,7 +104,7 @@ class Batch2:
             self.b.copy_(self.a)
         e1.record()
         e1.synchronize()
-        return 16.0 * self.S * self.n * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+        return 2.0 * self.a.element_size() * self.S * self.n * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
 
     def free(self):
         import torch
@@ -469,7 +471,7 @@ def roofline_only(a):
     conf = conf_of(w)
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
     if a.variant == "auto":
-        conv.autotune(S)
+        conv.autotune(S, a.dtype, a.dtype)
     else:
         # a fixed kernel: under counter collection the autotuner's timings are not the
         # kernels' own, so profiles name the kernel they measure
@@ -477,7 +479,7 @@ def roofline_only(a):
     init = conv.field("qinitial")
     g = w.gauss
     sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
-    bt = Batch2(conv, init, S)
+    bt = Batch2(conv, init, S, a.dtype)
     bt.run(2, conf.dtFine())
     torch.cuda.synchronize()
     ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -485,8 +487,9 @@ def roofline_only(a):
     ev[1].synchronize()
     k_ms = ev[0].elapsed_time(ev[1]) / a.reps
     copy = bt.copy_gbs(reps=3)
-    alg = 16.0 * S * N ** 3
-    print(json.dumps({"kernel_avg_ms": round(k_ms, 4), "bytes_per_launch": alg, "tile": conv.get_tile(S, True),
+    alg = 2.0 * {"f64": 8, "f32": 4, "bf16": 2}[a.dtype] * S * N ** 3
+    print(json.dumps({"kernel_avg_ms": round(k_ms, 4), "bytes_per_launch": alg, "dtype": a.dtype,
+                      "tile": conv.get_tile(S, a.dtype == "f64"),
                       "copy_bytes_read": alg / 2, "copy_bytes_written": alg / 2, "copy_gbs": round(copy, 1),
                       "reps": a.reps}), flush=True)
 
@@ -508,6 +511,9 @@ def main():
     ap.add_argument("--reps", type=int, default=5, help="launches of the kernel with --roofline-only")
     ap.add_argument("--variant", choices=["auto", "quad", "pair"], default="auto",
                     help="fused kernel for --roofline-only: autotuned (default) or a fixed 32x32x512 variant")
+    ap.add_argument("--dtype", choices=["f64", "f32", "bf16"], default="f64",
+                    help="storage type of the step with --roofline-only (f32 / bf16: FP32 arithmetic, "
+                         "the coarse propagator's kernels)")
     ap.add_argument("--slices", type=int, default=0,
                     help="slices per launch with --roofline-only (default: the workload's; 1: the "
                          "single-slice step of the coarse chain and the serial solve)")
