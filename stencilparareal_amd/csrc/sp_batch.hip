@@ -105,8 +105,9 @@ size_t arena_put(sp_batch* b, const void* data, size_t bytes)
     return off;
 }
 
-// Groups entries by tile shape (pair / 2x2 kernels need even, aligned fields: other entries take
-// the scalar 32x16x512 kernel) and appends the launches' records; returns the group index.
+// Groups entries by tile shape (pair / 2x2 kernels need even, aligned fields, the 2x2 kernel
+// FP64 storage: other entries take the scalar 32x16x512 kernel) and appends the launches'
+// records; returns the group index.
 int add_group(sp_batch* b, std::vector<Entry> ents, int idt, int odt, std::map<std::string, int>& memo,
               const std::string& key)
 {
@@ -114,8 +115,10 @@ int add_group(sp_batch* b, std::vector<Entry> ents, int idt, int odt, std::map<s
     if (hit != memo.end()) return hit->second;
     std::map<std::vector<int>, std::vector<Entry>> by_shape;
     for (Entry& e : ents) {
+        // the x-pair kernel takes any storage types, the 2x2 one FP64 only
+        const bool f64 = idt == SP_F64 && odt == SP_F64;
         if (e.tile.variant != SP_KERNEL_SCALAR &&
-            !(idt == SP_F64 && odt == SP_F64 && fused_record_pair_ok(e.st, e.in, e.out)))
+            !((f64 || e.tile.variant == SP_KERNEL_PAIR) && fused_record_pair_ok(e.st, e.in, e.out, idt, odt)))
             e.tile = FusedTile{32, 16, 512, e.tile.kchunk, SP_KERNEL_SCALAR};
         by_shape[{e.tile.variant, e.tile.tx, e.tile.ty, e.tile.nt}].push_back(e);
     }

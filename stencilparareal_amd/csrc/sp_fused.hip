@@ -1401,13 +1401,13 @@ __global__ void __launch_bounds__(NT) k_rk4_fused_grouped(GroupArgs g)
     rk4_scalar_body<SI, SO, T, TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
 }
 
-template <int TX, int TY, int NT>
+template <typename SI, typename SO, typename T, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_rk4_fused_pair_grouped(GroupArgs g)
 {
     const int bid = xcd_block(blockIdx.x, gridDim.x);
     const int e = group_entry(g, bid);
     ConstArgs* rec = (ConstArgs*)g.args + e;
-    rk4_pair_body<double, double, double, TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
+    rk4_pair_body<SI, SO, T, TX, TY, NT>(*(const FusedArgs*)rec, bid - g.first[e]);
 }
 
 __global__ void __launch_bounds__(QuadCfg::NT) k_rk4_fused_quad_grouped(GroupArgs g)
@@ -1570,6 +1570,23 @@ bool pair_ok(const FusedStep& st, View in, View out, int isz = 8, int osz = 8)
 }
 
 template <typename SI, typename SO, typename T>
+int launch_grouped_pair(hipStream_t s, const FusedTile& tile, GroupArgs g, long long nblocks)
+{
+    const dim3 grid((unsigned)nblocks);
+#define SP_PAIR_CASE(TX_, TY_, NT_)                                                                            \
+    if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                                                  \
+        hipLaunchKernelGGL((k_rk4_fused_pair_grouped<SI, SO, T, TX_, TY_, NT_>), grid, dim3(NT_), 0, s, g); \
+        SP_HIP(hipGetLastError());                                                                             \
+        return SP_OK;                                                                                          \
+    }
+    SP_PAIR_CASE(32, 32, 512)
+    SP_PAIR_CASE(32, 16, 512)
+    SP_PAIR_CASE(16, 16, 256)
+#undef SP_PAIR_CASE
+    return fail(SP_ERR_UNSUPPORTED, "grouped fused RK4 (pair): unsupported tile shape");
+}
+
+template <typename SI, typename SO, typename T>
 int launch_grouped_typed(hipStream_t s, const FusedTile& tile, GroupArgs g, long long nblocks)
 {
     const dim3 grid((unsigned)nblocks);
@@ -1603,7 +1620,10 @@ int fused_record_fill(void* rec, const FusedStep& st, const FusedTile& tile, int
     return SP_OK;
 }
 
-bool fused_record_pair_ok(const FusedStep& st, View in, View out) { return pair_ok(st, in, out); }
+bool fused_record_pair_ok(const FusedStep& st, View in, View out, int in_dtype, int out_dtype)
+{
+    return pair_ok(st, in, out, dtype_size(in_dtype), dtype_size(out_dtype));
+}
 
 int launch_rk4_grouped(hipStream_t s, const FusedTile& tile, int in_dtype, int out_dtype, const void* records,
                        const int* first, int n, long long nblocks)
@@ -1619,19 +1639,12 @@ int launch_rk4_grouped(hipStream_t s, const FusedTile& tile, int in_dtype, int o
         return SP_OK;
     }
     if (tile.variant == SP_KERNEL_PAIR) {
-        SP_CHECK(in_dtype == SP_F64 && out_dtype == SP_F64, SP_ERR_UNSUPPORTED, "grouped pair kernel is FP64 only");
-        const dim3 grid((unsigned)nblocks);
-#define SP_PAIR_CASE(TX_, TY_, NT_)                                                               \
-        if (tile.tx == TX_ && tile.ty == TY_ && tile.nt == NT_) {                                  \
-            hipLaunchKernelGGL((k_rk4_fused_pair_grouped<TX_, TY_, NT_>), grid, dim3(NT_), 0, s, g); \
-            SP_HIP(hipGetLastError());                                                             \
-            return SP_OK;                                                                          \
-        }
-        SP_PAIR_CASE(32, 32, 512)
-        SP_PAIR_CASE(32, 16, 512)
-        SP_PAIR_CASE(16, 16, 256)
-#undef SP_PAIR_CASE
-        return fail(SP_ERR_UNSUPPORTED, "grouped fused RK4 (pair): unsupported tile shape");
+        if (in_dtype == SP_F64 && out_dtype == SP_F64) return launch_grouped_pair<double, double, double>(s, tile, g, nblocks);
+        if (in_dtype == SP_F64 && out_dtype == SP_F32) return launch_grouped_pair<double, float, float>(s, tile, g, nblocks);
+        if (in_dtype == SP_F32 && out_dtype == SP_F32) return launch_grouped_pair<float, float, float>(s, tile, g, nblocks);
+        if (in_dtype == SP_F64 && out_dtype == SP_BF16) return launch_grouped_pair<double, bf16, float>(s, tile, g, nblocks);
+        if (in_dtype == SP_BF16 && out_dtype == SP_BF16) return launch_grouped_pair<bf16, bf16, float>(s, tile, g, nblocks);
+        return fail(SP_ERR_UNSUPPORTED, "grouped fused RK4 (pair): unsupported dtype pair");
     }
     if (in_dtype == SP_F64 && out_dtype == SP_F64) return launch_grouped_typed<double, double, double>(s, tile, g, nblocks);
     if (in_dtype == SP_F64 && out_dtype == SP_F32) return launch_grouped_typed<double, float, float>(s, tile, g, nblocks);

@@ -167,7 +167,7 @@ FusedTile default_tile(int ni, int nj, int nk, int nslices, int compute_f64);
 size_t fused_record_size();
 int fused_record_fill(void* rec, const FusedStep& st, const FusedTile& tile, int in_dtype, View in, int out_dtype,
                       View out, long long* nblocks);
-bool fused_record_pair_ok(const FusedStep& st, View in, View out);
+bool fused_record_pair_ok(const FusedStep& st, View in, View out, int in_dtype, int out_dtype);
 int launch_rk4_grouped(hipStream_t s, const FusedTile& tile, int in_dtype, int out_dtype, const void* records,
                        const int* first, int n, long long nblocks);
 // Tile shapes and k chunks the autotuner times for this size and batch.
