@@ -248,14 +248,16 @@ def cfg4(a):
 
 
 def parareal_solve(conv, init, conf, w, sol, serial, gdt):
-    """One tolerance-stopped Parareal solve with G in storage type `gdt`, driven iteration by
-    iteration (DoParallel, Parareal.h:110-171: begin = the initial coarse chain, iterations
-    until the increment test, the last slice's state), each iteration logged."""
+    """One tolerance-stopped Parareal solve with G in storage type `gdt` ("bf16:init64": BF16 G
+    with the initial coarse run in FP64), driven iteration by iteration (DoParallel,
+    Parareal.h:110-171: begin = the initial coarse chain, iterations until the increment
+    test, the last slice's state), each iteration logged."""
     import torch
     import stencilparareal_amd as sp
+    dt, _, init_mode = gdt.partition(":")
     log(f"Parareal solve to {w.tol:g}, G {gdt} ({w.substeps} sub-steps)")
-    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=gdt, substeps=w.substeps,
-                     tol=w.tol, use_graphs=True, autotune=True)
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=dt, substeps=w.substeps,
+                     tol=w.tol, use_graphs=True, autotune=True, initial_fp64=init_mode == "init64")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pr.begin()
@@ -501,8 +503,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the once-timed Parareal, serial and comparator runs")
-    ap.add_argument("--also-g", default="f64,f32",
-                    help="cfg4: G storage types solved beside the configured BF16 one (comma list; '' for none)")
+    ap.add_argument("--also-g", default="f64,bf16:init64,f32",
+                    help="cfg4: G storage types solved beside the configured BF16 one (comma list of f64, f32, "
+                         "bf16, with ':init64' for the initial coarse run in FP64; '' for none)")
     ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5"], default="cfg4",
                     help="cfg4: 384^3 x 64 slices, BF16 G (default); cfg2: 128^3 x 16 slices, FP64; "
                          "cfg5: the 24-problem uneven batch")

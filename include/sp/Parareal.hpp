@@ -31,6 +31,7 @@ struct PararealOptions {
     bool autotune = false;
     int correction = -1;  // -1: difference form for f32 / bf16 coarse storage, else the
                           // reference's; 0: the reference's; 1: difference form
+    bool initial_fp64 = false;  // f32 / bf16 coarse storage: the initial coarse run in FP64
 };
 
 // The communicator argument of the reference constructor: any value converts to it (an
@@ -74,6 +75,7 @@ public:
         c.autotune = opt.autotune ? 1 : 0;
         c.delta_correction = opt.correction < 0 ? (opt.coarse_dtype != SP_F64 ? 1 : 0) : opt.correction;
         c.use_graphs = opt.use_graphs ? 1 : 0;
+        c.initial_fp64 = opt.initial_fp64 ? 1 : 0;
         check(sp_parareal_create(propagator.handle(), &c, &pr_));
     }
     ~Parareal() { (void)sp_parareal_destroy(pr_); }

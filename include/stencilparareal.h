@@ -228,6 +228,10 @@ typedef struct sp_parareal_cfg {
                              (Parareal.h:19-35); 1: F(U_n^k) + G(U_n^{k+1} - U_n^k), equal for
                              this linear G, whose rounding then shrinks with the increment --
                              the form for SP_F32 / SP_BF16 coarse storage */
+    int initial_fp64;    /* 1 (with delta_correction and SP_F32 / SP_BF16 coarse storage): the
+                            initial coarse run U_n^0 = G^n(u0) in FP64, the iterations' coarse
+                            steps in coarse_dtype; 0: all in coarse_dtype (no counterpart in
+                            the reference, whose G is one type) */
 } sp_parareal_cfg;
 
 #define SP_MAX_ITERS 64

@@ -62,7 +62,7 @@ class PararealCfg(C.Structure):
                 ("steps_fine", C.c_int), ("steps_coarse", C.c_int), ("coarse_kind", C.c_int),
                 ("coarse_dtype", C.c_int), ("coarse_substeps", C.c_int), ("tol", C.c_double),
                 ("async_schedule", C.c_int), ("use_graphs", C.c_int), ("autotune", C.c_int),
-                ("delta_correction", C.c_int)]
+                ("delta_correction", C.c_int), ("initial_fp64", C.c_int)]
 
 
 class PararealReport(C.Structure):

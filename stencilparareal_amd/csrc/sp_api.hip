@@ -901,6 +901,7 @@ struct sp_parareal {
     int iters_done = 0;
     double u0_norm2 = 0.;  // sum of u0^2, for the blow-up check
     FusedTile tileF{}, tileG1{}, tileGS{};
+    FusedTile tileG1_64{};  // one-slice FP64 tile of an FP64 initial coarse run
     // With use_graphs an iteration is two graphs, the fine sweep and the coarse chain (its
     // coarse runs and corrections), launched back to back with events between them.
     hipGraphExec_t gexecF = nullptr, gexecC = nullptr;
@@ -936,11 +937,12 @@ FusedStep pr_step(const sp_parareal* pr, int nslices, double dt, const void* nud
 // times; step i of slice r runs at t0[r] + i*dtG (or at (first + i)*dtG from t = 0 for
 // the initial chain, first_index >= 0).  Input f64 (U) or gdt; output gdt in `dst`.
 int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View dst, View scr, const void* coef_base,
-               long long first_index, int steps)
+               long long first_index, int steps, int odt = -1)
 {
     sp_prop* p = pr->p;
     hipStream_t s = p->ctx->stream;
-    if (p->path == SP_PATH_STAGED && pr->cfg.coarse_kind == SP_COARSE_RK4 && pr->gdt == SP_F64 && idt == SP_F64) {
+    const int gdt = odt < 0 ? pr->gdt : odt;  // the chain's storage type (FP64 for an FP64 initial run)
+    if (p->path == SP_PATH_STAGED && pr->cfg.coarse_kind == SP_COARSE_RK4 && gdt == SP_F64 && idt == SP_F64) {
         // reference-order sweeps, slice by slice (src -> dst, then dst -> dst as DoRK4 does)
         for (int q = 0; q < nslices; ++q) {
             View a = src, b = dst;
@@ -973,15 +975,15 @@ int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View
             nud = (const char*)coef_base + ((size_t)i * pr->S + slice0) * coefs_size();
         }
         if (pr->cfg.coarse_kind == SP_COARSE_RK4) {
-            const FusedTile& tile = nslices == 1 ? pr->tileG1 : pr->tileGS;
-            SP_TRY(launch_rk4_fused(s, pr_step(pr, nslices, pr->dtG, nud, nu), tile, cdt, cur, pr->gdt, d));
+            const FusedTile& tile = gdt != pr->gdt ? pr->tileG1_64 : nslices == 1 ? pr->tileG1 : pr->tileGS;
+            SP_TRY(launch_rk4_fused(s, pr_step(pr, nslices, pr->dtG, nud, nu), tile, cdt, cur, gdt, d));
         } else {
             if (nud) return fail(SP_ERR_UNSUPPORTED, "batched Euler coarse with time-dependent nu");
             const StagedParams P{p->ni, p->nj, p->nk, p->cx, p->cy, p->cz, p->dx, p->dx * p->dx};
             SP_TRY(launch_euler_fused(s, P, cur, d, nu, pr->dtG, p->upwind, nslices));
         }
         cur = d;
-        cdt = pr->gdt;
+        cdt = gdt;
     }
     return SP_OK;
 }
@@ -1108,6 +1110,10 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
              "sp_parareal_create: Euler coarse propagator is SP_F64 only");
     SP_CHECK(cfg->coarse_substeps >= 1, SP_ERR_INVALID, "sp_parareal_create: coarse_substeps < 1");
     SP_CHECK(cfg->tol >= 0, SP_ERR_INVALID, "sp_parareal_create: tol < 0");
+    SP_CHECK(cfg->initial_fp64 == 0 || cfg->initial_fp64 == 1, SP_ERR_INVALID, "sp_parareal_create: initial_fp64");
+    SP_CHECK(!cfg->initial_fp64 || cfg->delta_correction, SP_ERR_UNSUPPORTED,
+             "sp_parareal_create: initial_fp64 needs the difference-form correction (the reference's form "
+             "subtracts coarse values that must come from one coarse propagator)");
     SP_CHECK(cfg->delta_correction == 0 || cfg->delta_correction == 1, SP_ERR_INVALID,
              "sp_parareal_create: delta_correction must be 0 or 1");
     SP_HIP(hipSetDevice(p->ctx->device));
@@ -1211,9 +1217,11 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
     pr->tileF = pick_tile(p, pr->S, 1);
     pr->tileG1 = pick_tile(p, 1, pr->gdt == SP_F64);
     pr->tileGS = pick_tile(p, pr->S, pr->gdt == SP_F64);
+    pr->tileG1_64 = pick_tile(p, 1, 1);
     {
         size_t wb = 0;
-        if (pr->tileF.variant == SP_KERNEL_SPLIT) wb = split_work_bytes(p->ni, p->nj, p->nk, pr->S);
+        if (pr->tileG1_64.variant == SP_KERNEL_SPLIT) wb = split_work_bytes(p->ni, p->nj, p->nk, 1);
+        if (pr->tileF.variant == SP_KERNEL_SPLIT) wb = std::max(wb, split_work_bytes(p->ni, p->nj, p->nk, pr->S));
         if (pr->gdt == SP_F64 && pr->tileG1.variant == SP_KERNEL_SPLIT)
             wb = std::max(wb, split_work_bytes(p->ni, p->nj, p->nk, 1));
         if (pr->gdt == SP_F64 && pr->tileGS.variant == SP_KERNEL_SPLIT)
@@ -1270,7 +1278,16 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
     // times differ, and they enter only through nu.  The chain then writes Gold itself and
     // runs one more segment for Gold_{S-1}; otherwise Gold is one batched run below.
     const bool gold_from_chain = !pr->cfg.delta_correction && pr->nu_const;
-    {
+    // FP64 initial run (initial_fp64): each segment from U_{r-1} straight into U_r; slice 0 of
+    // FS is the sub-step scratch.  (The difference form needs no coarse values.)
+    const bool init64 = pr->cfg.initial_fp64 && pr->cfg.delta_correction && pr->gdt != SP_F64;
+    if (init64) {
+        const View Z = cview(pr->FS.p, p->ni, p->nj);
+        for (int r = 1; r < pr->S; ++r)
+            SP_TRY(coarse_run(pr, 1, r - 1, SP_F64, cview((char*)pr->U.p + (size_t)(r - 1) * pr->n * 8, p->ni, p->nj),
+                              cview((char*)pr->U.p + (size_t)r * pr->n * 8, p->ni, p->nj), Z, nullptr,
+                              (long long)(r - 1) * pr->tsG, pr->tsG, SP_F64));
+    } else {
         const View bufs[2] = {cview(pr->Gw0.p, p->ni, p->nj), cview(pr->Gw1.p, p->ni, p->nj)};
         const View Z = cview(pr->FS.p, p->ni, p->nj);
         const auto gold = [&](int r) { return cview((char*)pr->Gold.p + (size_t)r * pr->n * gs, p->ni, p->nj); };
@@ -1321,7 +1338,11 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
         pr->u0_norm2 = n2;
     }
     // The coarse prediction of the final state is the last slice's coarse value.
-    if (pr->cfg.delta_correction) {
+    if (init64) {
+        SP_TRY(coarse_run(pr, 1, pr->S - 1, SP_F64, cview((char*)pr->U.p + (size_t)(pr->S - 1) * pr->n * 8, p->ni, p->nj),
+                          cview(pr->Q.p, p->ni, p->nj), cview(pr->FS.p, p->ni, p->nj), pr->coefG.p, -1, pr->tsG,
+                          SP_F64));
+    } else if (pr->cfg.delta_correction) {
         const View A = cview(pr->Gw0.p, p->ni, p->nj), B = cview(pr->Gw1.p, p->ni, p->nj);
         SP_TRY(coarse_run(pr, 1, pr->S - 1, SP_F64, cview((char*)pr->U.p + (size_t)(pr->S - 1) * pr->n * 8, p->ni, p->nj),
                           A, B, pr->coefG.p, -1, pr->tsG));

@@ -2,7 +2,8 @@
 // src/Parareal/main.cpp (Compare, Serial, Parallel, Timing), built on the drop-in C++ API
 // (include/sp/*.hpp) over libsp_hip.so.  The slices the reference spreads over MPI ranks
 // are the batch index of one device (--timeSlices).  Extensions: --coarse euler|rk4,
-// --coarseDtype f64|f32|bf16, --tol <t>, --graphs 0|1, --autotune 0|1.  Energy counters of the Cray nodes are not
+// --coarseDtype f64|f32|bf16, --initialFP64 0|1 (f32 / bf16 G: the initial coarse run in FP64),
+// --tol <t>, --graphs 0|1, --autotune 0|1.  Energy counters of the Cray nodes are not
 // available and not printed.
 #include <chrono>
 #include <cmath>
@@ -63,6 +64,7 @@ int main(int argc, char** argv)
         if (k == "--tol") opt.tol = std::atof(v.c_str());
         if (k == "--graphs") opt.use_graphs = v != "0";
         if (k == "--autotune") opt.autotune = v != "0";
+        if (k == "--initialFP64") opt.initial_fp64 = v != "0";
     }
 
     std::cout << "parareal: time-parallel advection-diffusion, all " << conf.timeSlices()

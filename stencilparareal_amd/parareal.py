@@ -27,12 +27,14 @@ class Parareal:
     correction: "standard" (the reference's G(U_n^{k+1}) + F(U_n^k) - G(U_n^k)), "delta"
     (F(U_n^k) + G(U_n^{k+1} - U_n^k), the same for this linear G, with G's rounding
     proportional to the increment) or "auto" (delta for f32 / bf16 coarse storage).
+    initial_fp64: with f32 / bf16 coarse storage and the delta correction, run the initial
+    coarse prediction U_n^0 = G^n(u0) in FP64 (the iterations' coarse steps keep coarse_dtype).
     """
 
     def __init__(self, propagator: Convection, initial: ConvectionField, solution: ConvectionField,
                  conf: RuntimeConfiguration, *, coarse: str = "euler", coarse_dtype: str = "f64",
                  substeps: int = 1, tol: float = 0.0, use_graphs: bool = False, autotune: bool = False,
-                 correction: str = "auto"):
+                 correction: str = "auto", initial_fp64: bool = False):
         self.prop = propagator
         self.qinitial = initial
         self.q = solution
@@ -53,6 +55,7 @@ class Parareal:
         if correction == "auto":
             correction = "standard" if coarse_dtype == "f64" else "delta"
         cfg.delta_correction = {"standard": 0, "delta": 1}[correction]
+        cfg.initial_fp64 = 1 if initial_fp64 else 0
         self.cfg = cfg
         h = C.c_void_p()
         L.call("sp_parareal_create", propagator.h, C.byref(cfg), C.byref(h))

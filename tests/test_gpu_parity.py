@@ -482,6 +482,32 @@ def test_parareal_split_tile_bitwise_and_vs_oracle(sp, graphs):
     assert relerr(outs[3], serial) <= 1e-12
 
 
+@pytest.mark.parametrize("gdt,factor", [("f32", 1.0), ("bf16", 0.1)])
+def test_parareal_initial_fp64_prediction(sp, gdt, factor):
+    # reduced-precision G with the initial coarse run in FP64: the prediction (k = 0) is the
+    # FP64-G Parareal's bit for bit, the iterates converge to the serial fine solution, and the
+    # first increment is below the all-reduced-precision run's (far below for BF16, whose
+    # rounding dominates the prediction's error here; FP32's does not)
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4", S=4, kmax=4)
+    init = conv.field("u0")
+    init.set(u0)
+    preds, incs = {}, {}
+    for name, kw in (("f64", dict(coarse_dtype="f64")), ("mixed", dict(coarse_dtype=gdt, initial_fp64=True)),
+                     ("plain", dict(coarse_dtype=gdt))):
+        sol = conv.field("sol_" + name)
+        pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", correction="delta", **kw)
+        pr.begin()
+        preds[name] = pr.solution().get()
+        incs[name] = [pr.iterate() for _ in range(conf.kmax)]
+        if name == "mixed":
+            assert O.error_reference(pr.solution().get(), serial) <= 1e-10
+    assert np.array_equal(preds["mixed"], preds["f64"])
+    assert incs["mixed"][0] <= factor * incs["plain"][0]
+    with pytest.raises(Exception):
+        sp.Parareal(conv, init, conv.field("x"), conf, coarse="rk4", coarse_dtype=gdt, correction="standard",
+                    initial_fp64=True)
+
+
 def test_parareal_autotuned_vs_oracle(sp):
     conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4")
     init, sol = conv.field(), conv.field()
