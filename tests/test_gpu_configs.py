@@ -3,7 +3,8 @@
 cfg1 (32^3, 4 slices, all FP64, T = 0.5): in full, every Parareal iterate against the
   oracle's Parareal, and the serial fine solution against the oracle's.
 cfg2 (128^3, 16 slices, FP64): on a shortened horizon (8 fine steps per slice, fine/coarse
-  step ratio kept), every iterate against the oracle.
+  step ratio kept), every iterate against the oracle; at the full horizon on a discrete
+  Fourier mode, every iterate and the serial fine solution against their closed forms.
 cfg3 (256^3, 32 slices, FP64 F, FP32 G) and cfg4 (384^3, 64 slices, FP64 F, BF16-storage G
   with 2 sub-steps, stop at a 1e-6 relative increment): at their full grid and slice
   count on a shortened horizon, through properties that do not depend on the size:
@@ -11,7 +12,8 @@ cfg3 (256^3, 32 slices, FP64 F, FP32 G) and cfg4 (384^3, 64 slices, FP64 F, BF16
   iterates against the GPU's serial FP64 fine solution falls with k to the FP64 floor
   (the reduced-precision G runs in the difference form of the correction), k = S
   reproduces that solution (cfg3), and the stopping rule ends the run with the error at
-  the level of the last increment (cfg4).
+  the level of the last increment (cfg4); and one slice's full fine horizon at 384^3 on a
+  discrete Fourier mode against its closed form.
 Random draws (Gaussian centres, widths, amplitudes, velocities) come from SplitMix64 with
 seed 0, as in the benchmark.
 """
@@ -152,3 +154,91 @@ def test_cfg4_full_size_short_horizon_stops_at_tol(sp):
     # more than 10x per iteration here)
     assert e <= 10 * incs[-1] + 1e-12
     del pr
+
+
+def _lam4(th, c, nu, dx):
+    # discrete Fourier symbol of Advection4 + nu * Laplace4 (pinned against the oracle's
+    # RHS4 in tests/test_oracle.py)
+    s = 0
+    for d in range(3):
+        t = th[d]
+        s += -1j * c[d] * (8 * np.sin(t) - np.sin(2 * t)) / (6 * dx)
+        s += nu * (-2 * np.cos(2 * t) + 32 * np.cos(t) - 30) / (12 * dx * dx)
+    return s
+
+
+def test_cfg2_full_horizon_fourier_mode_closed_form(sp):
+    # Config 2 at its full size and horizon (128^3, 16 slices, 20000 fine and 1008 coarse RK4
+    # steps over T = 1, K = 5) on a discrete Fourier mode, where every propagator is a complex
+    # scalar: F = R(dtF lam)^1250, G = R(dtG lam)^63 per slice.  The serial fine solution and
+    # every Parareal iterate (the scalar recursion U_{n+1}^{k+1} = G U_n^{k+1} + F U_n^k -
+    # G U_n^k) are closed forms, so the full horizon is checked without an oracle run.
+    w = workload(2)
+    conf = conf_of(w)
+    N, S = w.N, w.slices
+    dx = conf.dx()
+    m = (1, 2, 3)
+    th = [2 * np.pi * mm / N for mm in m]
+    k3, j3, i3 = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    v = np.exp(1j * (th[0] * i3 + th[1] * j3 + th[2] * k3))
+    lam = _lam4(th, w.c, w.nu, dx)
+
+    def R(z):
+        return 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+
+    tsF, tsG = conf.timeStepsFinePerTimeSlice(), conf.timeStepsCoarsePerTimeSlice()
+    F, G = R(conf.dtFine() * lam) ** tsF, R(conf.dtCoarse() * lam) ** tsG
+    U = [G ** n for n in range(S + 1)]  # k = 0: the initial coarse run
+    want = [U[S]]
+    for _ in range(conf.kmax):
+        new = [1.0 + 0j]
+        for n in range(S):
+            new.append(G * new[n] + F * U[n] - G * U[n])
+        U = new
+        want.append(U[S])
+    conv = sp.Convection(N, N, N, dx, w.nu, 0.0, *w.c)
+    init, sol = conv.field("qinitial"), conv.field("q")
+    init.set(np.ascontiguousarray(v.real))
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", use_graphs=True)
+    pr.begin()
+    errs = []
+    for k in range(conf.kmax + 1):
+        if k:
+            pr.iterate()
+        got = pr.solution().get()
+        ref = (want[k] * v).real
+        errs.append(float(np.abs(got - ref).max() / np.abs(ref).max()))
+    ser = conv.field("serial")
+    pr.q = ser
+    pr.DoSerial()
+    ref = ((F ** S) * v).real
+    e_ser = float(np.abs(ser.get() - ref).max() / np.abs(ref).max())
+    print("cfg2 Fourier mode, rel max error per iterate:", ["%.2e" % e for e in errs], "serial %.2e" % e_ser)
+    assert max(errs) <= 1e-11 and e_ser <= 1e-11
+    # the coarse prediction and the fine solution differ by 1.3e-8 for this mode, 1000x the
+    # tolerance: iterate 0 pins G, the later ones F and the correction
+    assert abs(want[1] - want[0]) > 1e-9 * abs(want[0])
+
+
+def test_cfg4_one_slice_full_fine_horizon_closed_form(sp):
+    # Config 4's fine propagator over one whole slice (1563 FP64 RK4 steps of 2e-5 on 384^3,
+    # the steps every slice runs per iteration) on a discrete Fourier mode, against R(z)^1563
+    w = workload(4)
+    conf = conf_of(w)
+    N = w.N
+    dx = conf.dx()
+    th = [2 * np.pi * mm / N for mm in (2, 1, 3)]
+    k3, j3, i3 = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    v = np.exp(1j * (th[0] * i3 + th[1] * j3 + th[2] * k3))
+    del k3, j3, i3
+    z = conf.dtFine() * _lam4(th, w.c, w.nu, dx)
+    steps = conf.timeStepsFinePerTimeSlice()
+    amp = (1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24) ** steps
+    conv = sp.Convection(N, N, N, dx, w.nu, 0.0, *w.c)
+    a, b = conv.field("a"), conv.field("b")
+    a.set(np.ascontiguousarray(v.real))
+    conv.DoRK4(a, b, 0.0, conf.dtFine(), steps)
+    ref = (amp * v).real
+    e = float(np.abs(b.get() - ref).max() / np.abs(ref).max())
+    print(f"cfg4 one slice, {steps} fine steps: rel max error {e:.2e}")
+    assert e <= 1e-11
