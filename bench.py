@@ -503,7 +503,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the once-timed Parareal, serial and comparator runs")
-    ap.add_argument("--also-g", default="f64,bf16:init64,f32",
+    ap.add_argument("--also-g", default="f64,bf16:init64",
                     help="cfg4: G storage types solved beside the configured BF16 one (comma list of f64, f32, "
                          "bf16, with ':init64' for the initial coarse run in FP64; '' for none)")
     ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5"], default="cfg4",
