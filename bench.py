@@ -249,15 +249,17 @@ def cfg4(a):
 
 def parareal_solve(conv, init, conf, w, sol, serial, gdt):
     """One tolerance-stopped Parareal solve with G in storage type `gdt` ("bf16:init64": BF16 G
-    with the initial coarse run in FP64), driven iteration by iteration (DoParallel,
+    with the initial coarse run in FP64; "f64:spectral": the FP64 RK4 coarse map applied as its
+    Fourier multiplier), driven iteration by iteration (DoParallel,
     Parareal.h:110-171: begin = the initial coarse chain, iterations until the increment
     test, the last slice's state), each iteration logged."""
     import torch
     import stencilparareal_amd as sp
-    dt, _, init_mode = gdt.partition(":")
+    dt, _, mode = gdt.partition(":")
     log(f"Parareal solve to {w.tol:g}, G {gdt} ({w.substeps} sub-steps)")
-    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=dt, substeps=w.substeps,
-                     tol=w.tol, use_graphs=True, autotune=True, initial_fp64=init_mode == "init64")
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4_spectral" if mode == "spectral" else "rk4",
+                     coarse_dtype=dt, substeps=w.substeps, tol=w.tol, use_graphs=True, autotune=True,
+                     initial_fp64=mode == "init64")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pr.begin()
@@ -310,7 +312,8 @@ def cfg4_solve(a, w, conf, conv, init):
     # times: K iterations of S slices x tsF reference-order fine steps, S-1 coarse segments
     # of tsG FP64 reference-order RK4 sub-steps, plus the initial coarse chain (S segments);
     # K = the iterations FP64 G takes on this problem (measured above when run)
-    K_plain = runs["f64"]["iterations"] if "f64" in runs else main["iterations"]
+    f64_runs = [g for g in runs if g.split(":")[0] == "f64"]
+    K_plain = runs[f64_runs[0]]["iterations"] if f64_runs else main["iterations"]
     log("plain comparator: reference-order step times")
     tsG = conf.timeStepsCoarsePerTimeSlice() * w.substeps
     dtG = conf.dtCoarse() / w.substeps
@@ -326,7 +329,7 @@ def cfg4_solve(a, w, conf, conv, init):
         "speedup_vs_serial_fine": round(t_serial / t_par, 4),
         "plain_fp64_parareal_reference_structure": {
             "wall_s_extrapolated": round(plain_s, 1), "iterations": K_plain,
-            "iterations_from": "the FP64-G solve above" if "f64" in runs else "the configured solve",
+            "iterations_from": f"the {f64_runs[0]} solve above" if f64_runs else "the configured solve",
             "fine_step_ms": round(f_ms, 4), "coarse_substep_ms": round(g_ms, 4),
             "how": "reference-order sweeps (4 RHS + 3 Euler + 1 RK kernels per step), one slice after another, "
                    "FP64 G; measured step times x the solve's step counts"},
@@ -377,6 +380,26 @@ def cfg2(a):
     torch.cuda.synchronize()
     t_par = time.perf_counter() - t0
     K = rep.iterations
+    err = sp.rel_l2(sol, serial)
+    del pr
+    # the same solve with the coarse RK4 map applied as its Fourier multiplier per segment
+    sol2 = conv.field("q_spectral")
+    pr2 = sp.Parareal(conv, init, sol2, conf, coarse="rk4_spectral", use_graphs=True, autotune=True)
+    pr2.DoParallel()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep2 = pr2.DoParallel()
+    torch.cuda.synchronize()
+    t_spec = time.perf_counter() - t0
+    K2 = rep2.iterations
+    spectral = {"wall_s": round(t_spec, 4), "iterations": K2,
+                "per_iteration": [{"fine_s": round(rep2.t_fine[k], 4), "chain_s": round(rep2.t_coarse[k], 4)}
+                                  for k in range(K2)],
+                "iteration_over_fine_sweep": round(sum(rep2.t_fine[k] + rep2.t_coarse[k] for k in range(K2)) /
+                                                   sum(rep2.t_fine[k] for k in range(K2)), 4),
+                "rel_l2_err": float(f"{sp.rel_l2(sol2, serial):.3e}"),
+                "rel_diff_vs_stencil_g": float(f"{sp.rel_l2(sol2, sol):.3e}")}
+    del pr2
     res = {"metric": METRIC, "value": round(lup / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -390,7 +413,8 @@ def cfg2(a):
                                           for k in range(K)],
                         "iteration_over_fine_sweep": round(sum(rep.t_fine[k] + rep.t_coarse[k] for k in range(K)) /
                                                            sum(rep.t_fine[k] for k in range(K)), 4),
-                        "rel_l2_err": float(f"{sp.rel_l2(sol, serial):.3e}")},
+                        "rel_l2_err": float(f"{err:.3e}")},
+           "parareal_spectral_g": spectral,
            "serial_fine_wall_s": round(t_serial, 4), "cpu_baseline": None}
     print(json.dumps(res), flush=True)
 
@@ -503,9 +527,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the once-timed Parareal, serial and comparator runs")
-    ap.add_argument("--also-g", default="f64,bf16:init64",
-                    help="cfg4: G storage types solved beside the configured BF16 one (comma list of f64, f32, "
-                         "bf16, with ':init64' for the initial coarse run in FP64; '' for none)")
+    ap.add_argument("--also-g", default="f64:spectral,bf16:init64",
+                    help="cfg4: G variants solved beside the configured BF16 one (comma list of f64, f32, bf16; "
+                         "':init64' for the initial coarse run in FP64, 'f64:spectral' for the FP64 RK4 coarse "
+                         "map as a Fourier multiplier; '' for none)")
     ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5"], default="cfg4",
                     help="cfg4: 384^3 x 64 slices, BF16 G (default); cfg2: 128^3 x 16 slices, FP64; "
                          "cfg5: the 24-problem uneven batch")

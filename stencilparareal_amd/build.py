@@ -32,6 +32,7 @@ SOURCES = {
     "sp_misc.hip": ["-ffp-contract=off"],
     "sp_fused.hip": [],
     "sp_batch.hip": [],
+    "sp_spectral.hip": [],
 }
 HEADERS = ["sp_internal.hpp", "sp_handles.hpp", os.path.join("..", "..", "include", "stencilparareal.h")]
 
@@ -79,7 +80,10 @@ def build(force: bool = False, verbose: bool = False, jobs: int = 4) -> str:
     _wait(procs)
     if force or _stale(LIB, objs):
         tmp = LIB + ".tmp"
-        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+        # hipFFT (the spectral coarse propagator), found through the ROCm tree at run time
+        rocm_lib = os.path.join(os.path.dirname(os.path.dirname(cc)), "lib")
+        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + \
+              ["-L", rocm_lib, "-lhipfft", f"-Wl,-rpath,{rocm_lib}"]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

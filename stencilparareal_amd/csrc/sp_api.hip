@@ -895,6 +895,8 @@ struct sp_parareal {
     Buf U, F, FS, Gold, Gw0, Gw1, Q, coefF, coefG, part;
     Buf D;  // difference form: U_n^{k+1} - U_n^k of the chain's current slice
     Buf W;  // work field of split-tile steps (its own: captured graphs keep its address)
+    SpectralPlan* spec = nullptr;  // SP_COARSE_RK4_SPECTRAL: FFT plans and coefficient buffer
+    Buf nu_spec;  // ... and its per-sub-step nu: [0, S*tsG) from t = 0, then [i*S + r] per slice
     std::vector<double> nuF_host, nuG_host;
     bool nu_const = true;
     bool begun = false;
@@ -942,6 +944,28 @@ int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View
     sp_prop* p = pr->p;
     hipStream_t s = p->ctx->stream;
     const int gdt = odt < 0 ? pr->gdt : odt;  // the chain's storage type (FP64 for an FP64 initial run)
+    if (pr->cfg.coarse_kind == SP_COARSE_RK4_SPECTRAL) {
+        // each slice's segment as one Fourier multiplier (compact FP64 fields)
+        SP_CHECK(idt == SP_F64 && gdt == SP_F64 && src.sj == p->ni && dst.sj == p->ni &&
+                     src.sk == (long long)p->ni * p->nj && dst.sk == (long long)p->ni * p->nj,
+                 SP_ERR_INVALID, "spectral coarse propagator: compact FP64 fields only");
+        const long long per = (long long)pr->S * pr->tsG;
+        for (int q = 0; q < nslices; ++q) {
+            SpectralStep st{p->dx, p->cx, p->cy, p->cz, pr->dtG, steps, p->nu0, nullptr, 1};
+            if (!pr->nu_const) {
+                const double* tab = (const double*)pr->nu_spec.p;
+                if (first_index >= 0) {
+                    st.nu_tab = tab + first_index;
+                } else {
+                    st.nu_tab = tab + per + slice0 + q;
+                    st.nu_stride = pr->S;
+                }
+            }
+            SP_TRY(spectral_apply(pr->spec, s, (const double*)src.p + (size_t)q * src.slice,
+                                  (double*)dst.p + (size_t)q * dst.slice, st));
+        }
+        return SP_OK;
+    }
     if (p->path == SP_PATH_STAGED && pr->cfg.coarse_kind == SP_COARSE_RK4 && gdt == SP_F64 && idt == SP_F64) {
         // reference-order sweeps, slice by slice (src -> dst, then dst -> dst as DoRK4 does)
         for (int q = 0; q < nslices; ++q) {
@@ -1102,8 +1126,11 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
     SP_CHECK(cfg->kmax >= 0 && cfg->kmax <= SP_MAX_ITERS, SP_ERR_INVALID, "sp_parareal_create: kmax out of range");
     SP_CHECK(cfg->end_time > 0 && std::isfinite(cfg->end_time), SP_ERR_INVALID, "sp_parareal_create: end_time");
     SP_CHECK(cfg->steps_fine >= 1 && cfg->steps_coarse >= 1, SP_ERR_INVALID, "sp_parareal_create: step counts");
-    SP_CHECK(cfg->coarse_kind == SP_COARSE_EULER || cfg->coarse_kind == SP_COARSE_RK4, SP_ERR_INVALID,
-             "sp_parareal_create: coarse_kind");
+    SP_CHECK(cfg->coarse_kind == SP_COARSE_EULER || cfg->coarse_kind == SP_COARSE_RK4 ||
+                 cfg->coarse_kind == SP_COARSE_RK4_SPECTRAL,
+             SP_ERR_INVALID, "sp_parareal_create: coarse_kind");
+    SP_CHECK(cfg->coarse_kind != SP_COARSE_RK4_SPECTRAL || cfg->coarse_dtype == SP_F64, SP_ERR_UNSUPPORTED,
+             "sp_parareal_create: the spectral coarse propagator runs in FP64");
     SP_CHECK(cfg->coarse_dtype == SP_F64 || cfg->coarse_dtype == SP_F32 || cfg->coarse_dtype == SP_BF16,
              SP_ERR_INVALID, "sp_parareal_create: coarse_dtype");
     SP_CHECK(cfg->coarse_kind == SP_COARSE_RK4 || cfg->coarse_dtype == SP_F64, SP_ERR_UNSUPPORTED,
@@ -1174,6 +1201,24 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
         if (r == SP_OK && (hipMemcpy(pr->coefF.p, cf.data(), cf.size(), hipMemcpyHostToDevice) != hipSuccess ||
                            hipMemcpy(pr->coefG.p, cg.data(), cg.size(), hipMemcpyHostToDevice) != hipSuccess))
             r = fail(SP_ERR_HIP, "sp_parareal_create: coefficient table upload");
+        if (r != SP_OK) {
+            sp_parareal_destroy(pr);
+            return r;
+        }
+    }
+    if (cfg->coarse_kind == SP_COARSE_RK4_SPECTRAL) {
+        r = spectral_create(&pr->spec, p->ni, p->nj, p->nk);
+        if (r == SP_OK && !pr->nu_const) {
+            // nu of every coarse sub-step: from t = 0 (the initial run), then per slice start
+            const size_t per = (size_t)pr->S * pr->tsG;
+            std::vector<double> tab(2 * per);
+            for (size_t j = 0; j < per; ++j) tab[j] = computenu(p, 0. + (double)j * pr->dtG);
+            std::copy(pr->nuG_host.begin(), pr->nuG_host.end(), tab.begin() + per);
+            chk(pr->nu_spec.ensure(tab.size() * sizeof(double)));
+            if (r == SP_OK && hipMemcpy(pr->nu_spec.p, tab.data(), tab.size() * sizeof(double),
+                                        hipMemcpyHostToDevice) != hipSuccess)
+                r = fail(SP_ERR_HIP, "sp_parareal_create: nu table upload");
+        }
         if (r != SP_OK) {
             sp_parareal_destroy(pr);
             return r;
@@ -1250,8 +1295,9 @@ int sp_parareal_destroy(sp_parareal* pr)
     for (hipEvent_t e : pr->evU)
         if (e) (void)hipEventDestroy(e);
     for (Buf* b : {&pr->U, &pr->F, &pr->FS, &pr->Gold, &pr->Gw0, &pr->Gw1, &pr->Q, &pr->coefF, &pr->coefG, &pr->part,
-                   &pr->D, &pr->W})
+                   &pr->D, &pr->W, &pr->nu_spec})
         b->release();
+    spectral_destroy(pr->spec);
     sp_prop* p = pr->p;
     delete pr;
     prop_release(p);
@@ -1358,6 +1404,14 @@ int sp_parareal_begin(sp_parareal* pr, const sp_layout* l0, const void* u0)
     return SP_OK;
 }
 
+// The context's stream set to s for a scope (work launched beside a captured graph).
+struct StreamScope {
+    sp_ctx* c;
+    hipStream_t keep;
+    StreamScope(sp_ctx* c_, hipStream_t s) : c(c_), keep(c_->stream) { c->stream = s; }
+    ~StreamScope() { c->stream = keep; }
+};
+
 // Captures `body` (enqueued on stream s) as a graph and instantiates it.
 static int capture(sp_parareal* pr, hipStream_t s, hipGraph_t* graph, hipGraphExec_t* exec,
                    const std::function<int()>& body)
@@ -1395,16 +1449,25 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
             SP_HIP(hipStreamSynchronize(nullptr));
             s = p->ctx->own;
         }
+        // The spectral coarse chain (S FFT pairs) is launched, not captured: its FFT library
+        // calls stay out of the graph.
+        const bool chain_graph = pr->cfg.coarse_kind != SP_COARSE_RK4_SPECTRAL;
         if (!pr->gexecF) {
             SP_TRY(capture(pr, s, &pr->graphF, &pr->gexecF, [&] { return fine_sweep(pr); }));
-            SP_TRY(capture(pr, s, &pr->graphC, &pr->gexecC, [&] {
-                return pr->cfg.delta_correction ? coarse_chain_delta(pr, false) : coarse_chain(pr, false);
-            }));
+            if (chain_graph)
+                SP_TRY(capture(pr, s, &pr->graphC, &pr->gexecC, [&] {
+                    return pr->cfg.delta_correction ? coarse_chain_delta(pr, false) : coarse_chain(pr, false);
+                }));
         }
         SP_HIP(hipEventRecord(pr->ev[0], s));
         SP_HIP(hipGraphLaunch(pr->gexecF, s));
         SP_HIP(hipEventRecord(pr->ev[1], s));
-        SP_HIP(hipGraphLaunch(pr->gexecC, s));
+        if (chain_graph) {
+            SP_HIP(hipGraphLaunch(pr->gexecC, s));
+        } else {
+            const StreamScope scope(p->ctx, s);
+            SP_TRY((pr->cfg.delta_correction ? coarse_chain_delta(pr, false) : coarse_chain(pr, false)));
+        }
         SP_HIP(hipEventRecord(pr->ev[2], s));
     } else {
         if (pr->evU.empty()) {

@@ -148,6 +148,22 @@ struct FusedStep {
     void* work = nullptr;  // SP_KERNEL_SPLIT: FP64 field of nslices compact slices (v2)
     size_t work_bytes = 0;
 };
+// The RK4 coarse propagator as a Fourier multiplier (sp_spectral.hip): one segment of
+// `steps` sub-steps of length h on a compact FP64 field, nu per sub-step from a device table
+// (nu_tab[s * nu_stride]) or constant (nu_tab == nullptr).
+struct SpectralStep {
+    double dx, cx, cy, cz;
+    double h;
+    int steps;
+    double nu;
+    const double* nu_tab;
+    long long nu_stride;
+};
+struct SpectralPlan;
+int spectral_create(SpectralPlan** out, int ni, int nj, int nk);
+void spectral_destroy(SpectralPlan* sp);
+int spectral_apply(SpectralPlan* sp, hipStream_t s, const double* in, double* out, const SpectralStep& st);
+
 // Bytes of the work field a SP_KERNEL_SPLIT step over `nslices` slices needs.
 inline size_t split_work_bytes(int ni, int nj, int nk, int nslices)
 {

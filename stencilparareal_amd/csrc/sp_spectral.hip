@@ -1,0 +1,156 @@
+// sp_spectral.hip -- the RK4 coarse propagator applied as its exact Fourier multiplier.
+//
+// On the periodic grid the stencil operator L = Advection4 + nu Laplace4 is diagonal in the
+// discrete Fourier basis: L e^{i theta.x} = lambda(theta) e^{i theta.x} with
+//   lambda = sum_d [ -i c_d (8 sin th_d - sin 2 th_d) / (6 dx)
+//                    + nu (-2 cos 2 th_d + 32 cos th_d - 30) / (12 dx^2) ],
+// the symbol of ConvectionFunctions.h:66-104 and :135-166 (Advection4, Laplace4).  One
+// fused RK4 step multiplies each mode by R(h lambda) = 1 + z (1 + z/2 (1 + z/3 (1 + z/4))),
+// z = h lambda, the polynomial of the nested stages; a coarse segment of n sub-steps by the
+// product of its n factors (nu(t) may change from step to step, never in space).  So a whole
+// segment G is one real-to-complex FFT (hipFFT), one pointwise product, one inverse FFT: the
+// same linear map as n stencil steps, to rounding, at a cost that does not grow with n.
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <string>
+
+#include "sp_internal.hpp"
+
+namespace sp {
+
+struct SpectralPlan {
+    int ni = 0, nj = 0, nk = 0;
+    hipfftHandle fwd = 0, inv = 0;
+    double2* X = nullptr;  // nk x nj x (ni/2 + 1) complex coefficients
+};
+
+namespace {
+
+const char* fft_err(hipfftResult r)
+{
+    switch (r) {
+    case HIPFFT_SUCCESS: return "success";
+    case HIPFFT_INVALID_PLAN: return "invalid plan";
+    case HIPFFT_ALLOC_FAILED: return "allocation failed";
+    case HIPFFT_INVALID_VALUE: return "invalid value";
+    case HIPFFT_EXEC_FAILED: return "execution failed";
+    case HIPFFT_SETUP_FAILED: return "setup failed";
+    case HIPFFT_INVALID_SIZE: return "invalid size";
+    default: return "error";
+    }
+}
+
+#define SP_FFT(call)                                                                                  \
+    do {                                                                                              \
+        const hipfftResult r_ = (call);                                                               \
+        if (r_ != HIPFFT_SUCCESS) return fail(SP_ERR_HIP, std::string("hipFFT: ") + fft_err(r_) + " (" #call ")"); \
+    } while (0)
+
+struct Cplx {
+    double re, im;
+};
+__device__ __forceinline__ Cplx cmul(Cplx a, Cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+
+// R(z) in the nested form of the fused stages
+__device__ __forceinline__ Cplx rk4_factor(double zr, double zi)
+{
+    Cplx p{1. + zr * 0.25, zi * 0.25};                      // 1 + z/4
+    Cplx t = cmul({zr / 3., zi / 3.}, p);                   // z/3 (1 + z/4)
+    p = {1. + t.re, t.im};
+    t = cmul({zr * 0.5, zi * 0.5}, p);
+    p = {1. + t.re, t.im};
+    t = cmul({zr, zi}, p);
+    return {1. + t.re, t.im};
+}
+
+// One thread per coefficient: multiply by prod_i R(h lambda(theta; nu_i)) and the inverse
+// transform's 1/N.  nu_tab == nullptr: constant nu, the product by binary powering.
+__global__ void __launch_bounds__(256) k_spectral_rk4(double2* __restrict__ X, int ni, int nj, int nk, SpectralStep st,
+                                                      double scale)
+{
+    const int nic = ni / 2 + 1;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nic * nj * nk;
+    if (idx >= total) return;
+    const int i = (int)(idx % nic);
+    const int j = (int)((idx / nic) % nj);
+    const int k = (int)(idx / ((long long)nic * nj));
+    const double two_pi = 6.283185307179586476925286766559;
+    const double th[3] = {two_pi * i / ni, two_pi * j / nj, two_pi * k / nk};
+    const double c[3] = {st.cx, st.cy, st.cz};
+    double a_im = 0., b = 0.;
+    for (int d = 0; d < 3; ++d) {
+        double s1, c1, s2, c2;
+        sincos(th[d], &s1, &c1);
+        sincos(2. * th[d], &s2, &c2);
+        a_im -= c[d] * (8. * s1 - s2) / (6. * st.dx);
+        b += (-2. * c2 + 32. * c1 - 30.) / (12. * st.dx * st.dx);
+    }
+    Cplx m{1., 0.};
+    if (st.nu_tab) {
+        for (int s = 0; s < st.steps; ++s) {
+            const double nu = st.nu_tab[(long long)s * st.nu_stride];
+            m = cmul(m, rk4_factor(st.h * nu * b, st.h * a_im));
+        }
+    } else {
+        Cplx r = rk4_factor(st.h * st.nu * b, st.h * a_im);
+        for (int n = st.steps; n > 0; n >>= 1) {
+            if (n & 1) m = cmul(m, r);
+            r = cmul(r, r);
+        }
+    }
+    const double2 x = X[idx];
+    X[idx] = make_double2(scale * (x.x * m.re - x.y * m.im), scale * (x.x * m.im + x.y * m.re));
+}
+
+}  // namespace
+
+int spectral_create(SpectralPlan** out, int ni, int nj, int nk)
+{
+    SpectralPlan* sp = new SpectralPlan;
+    sp->ni = ni;
+    sp->nj = nj;
+    sp->nk = nk;
+    int r = SP_OK;
+    const size_t coefs = (size_t)nk * nj * (ni / 2 + 1);
+    if (hipMalloc(&sp->X, coefs * sizeof(double2)) != hipSuccess) r = fail(SP_ERR_NOMEM, "spectral G: coefficient buffer");
+    if (r == SP_OK && hipfftPlan3d(&sp->fwd, nk, nj, ni, HIPFFT_D2Z) != HIPFFT_SUCCESS)
+        r = fail(SP_ERR_HIP, "spectral G: hipfftPlan3d (D2Z)");
+    if (r == SP_OK && hipfftPlan3d(&sp->inv, nk, nj, ni, HIPFFT_Z2D) != HIPFFT_SUCCESS)
+        r = fail(SP_ERR_HIP, "spectral G: hipfftPlan3d (Z2D)");
+    if (r != SP_OK) {
+        spectral_destroy(sp);
+        return r;
+    }
+    *out = sp;
+    return SP_OK;
+}
+
+void spectral_destroy(SpectralPlan* sp)
+{
+    if (!sp) return;
+    if (sp->fwd) (void)hipfftDestroy(sp->fwd);
+    if (sp->inv) (void)hipfftDestroy(sp->inv);
+    if (sp->X) (void)hipFree(sp->X);
+    delete sp;
+}
+
+int spectral_apply(SpectralPlan* sp, hipStream_t s, const double* in, double* out, const SpectralStep& st)
+{
+    SP_CHECK(sp && in && out, SP_ERR_INVALID, "spectral G: null argument");
+    SP_CHECK(st.steps >= 0, SP_ERR_INVALID, "spectral G: steps < 0");
+    SP_FFT(hipfftSetStream(sp->fwd, s));
+    SP_FFT(hipfftSetStream(sp->inv, s));
+    SP_FFT(hipfftExecD2Z(sp->fwd, const_cast<double*>(in), (hipfftDoubleComplex*)sp->X));
+    const long long total = (long long)sp->nk * sp->nj * (sp->ni / 2 + 1);
+    const double scale = 1.0 / ((double)sp->ni * sp->nj * sp->nk);
+    hipLaunchKernelGGL(k_spectral_rk4, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, sp->X, sp->ni, sp->nj,
+                       sp->nk, st, scale);
+    SP_HIP(hipGetLastError());
+    SP_FFT(hipfftExecZ2D(sp->inv, (hipfftDoubleComplex*)sp->X, out));
+    return SP_OK;
+}
+
+}  // namespace sp

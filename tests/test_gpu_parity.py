@@ -508,6 +508,27 @@ def test_parareal_initial_fp64_prediction(sp, gdt, factor):
                     initial_fp64=True)
 
 
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("nufreq", [0.0, 80.0])
+def test_parareal_spectral_coarse_vs_oracle(sp, graphs, nufreq):
+    # the RK4 coarse propagator applied as its Fourier multiplier: the prediction (k = 0) is
+    # the stencil RK4 chain's to rounding, and every iterate the oracle's (standard form,
+    # stencil RK4 G) within 1e-12; with nu(t) the multiplier is the product of each
+    # sub-step's factor
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4", N=16, nufreq=nufreq)
+    init = conv.field("u0")
+    init.set(u0)
+    sol = conv.field("sol")
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4_spectral", use_graphs=graphs)
+    pr.begin()
+    assert relerr(pr.solution().get(), iters[0]) <= 1e-13
+    for k in range(1, conf.kmax + 1):
+        pr.iterate()
+        assert relerr(pr.solution().get(), iters[k]) <= 1e-12, k
+    with pytest.raises(Exception):
+        sp.Parareal(conv, init, sol, conf, coarse="rk4_spectral", coarse_dtype="f32")
+
+
 def test_parareal_autotuned_vs_oracle(sp):
     conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4")
     init, sol = conv.field(), conv.field()
