@@ -24,6 +24,9 @@ struct SpectralPlan {
     int ni = 0, nj = 0, nk = 0;
     hipfftHandle fwd = 0, inv = 0;
     double2* X = nullptr;  // nk x nj x (ni/2 + 1) complex coefficients
+    double2* M = nullptr;  // the multiplier of the last constant-nu segment (scale included)
+    SpectralStep m_key{};  // ... and the segment it belongs to
+    bool m_valid = false;
 };
 
 namespace {
@@ -65,10 +68,11 @@ __device__ __forceinline__ Cplx rk4_factor(double zr, double zi)
     return {1. + t.re, t.im};
 }
 
-// One thread per coefficient: multiply by prod_i R(h lambda(theta; nu_i)) and the inverse
-// transform's 1/N.  nu_tab == nullptr: constant nu, the product by binary powering.
-__global__ void __launch_bounds__(256) k_spectral_rk4(double2* __restrict__ X, int ni, int nj, int nk, SpectralStep st,
-                                                      double scale)
+// One thread per coefficient: the multiplier prod_i R(h lambda(theta; nu_i)) times the inverse
+// transform's 1/N, applied to X (M == nullptr) or stored in M (X == nullptr).  nu_tab ==
+// nullptr: constant nu, the product by binary powering.
+__global__ void __launch_bounds__(256) k_spectral_rk4(double2* __restrict__ X, double2* __restrict__ M, int ni, int nj,
+                                                      int nk, SpectralStep st, double scale)
 {
     const int nic = ni / 2 + 1;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,8 +105,29 @@ __global__ void __launch_bounds__(256) k_spectral_rk4(double2* __restrict__ X, i
             r = cmul(r, r);
         }
     }
+    m = {scale * m.re, scale * m.im};
+    if (M) {
+        M[idx] = make_double2(m.re, m.im);
+        return;
+    }
     const double2 x = X[idx];
-    X[idx] = make_double2(scale * (x.x * m.re - x.y * m.im), scale * (x.x * m.im + x.y * m.re));
+    X[idx] = make_double2(x.x * m.re - x.y * m.im, x.x * m.im + x.y * m.re);
+}
+
+// X *= M, coefficient by coefficient (a constant-nu segment whose multiplier is kept)
+__global__ void __launch_bounds__(256) k_spectral_apply(double2* __restrict__ X, const double2* __restrict__ M,
+                                                        long long total)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const double2 x = X[idx], m = M[idx];
+    X[idx] = make_double2(x.x * m.x - x.y * m.y, x.x * m.y + x.y * m.x);
+}
+
+bool same_segment(const SpectralStep& a, const SpectralStep& b)
+{
+    return a.dx == b.dx && a.cx == b.cx && a.cy == b.cy && a.cz == b.cz && a.h == b.h && a.steps == b.steps &&
+           a.nu == b.nu;
 }
 
 }  // namespace
@@ -115,7 +140,8 @@ int spectral_create(SpectralPlan** out, int ni, int nj, int nk)
     sp->nk = nk;
     int r = SP_OK;
     const size_t coefs = (size_t)nk * nj * (ni / 2 + 1);
-    if (hipMalloc(&sp->X, coefs * sizeof(double2)) != hipSuccess) r = fail(SP_ERR_NOMEM, "spectral G: coefficient buffer");
+    if (hipMalloc(&sp->X, coefs * sizeof(double2)) != hipSuccess || hipMalloc(&sp->M, coefs * sizeof(double2)) != hipSuccess)
+        r = fail(SP_ERR_NOMEM, "spectral G: coefficient buffers");
     if (r == SP_OK && hipfftPlan3d(&sp->fwd, nk, nj, ni, HIPFFT_D2Z) != HIPFFT_SUCCESS)
         r = fail(SP_ERR_HIP, "spectral G: hipfftPlan3d (D2Z)");
     if (r == SP_OK && hipfftPlan3d(&sp->inv, nk, nj, ni, HIPFFT_Z2D) != HIPFFT_SUCCESS)
@@ -134,6 +160,7 @@ void spectral_destroy(SpectralPlan* sp)
     if (sp->fwd) (void)hipfftDestroy(sp->fwd);
     if (sp->inv) (void)hipfftDestroy(sp->inv);
     if (sp->X) (void)hipFree(sp->X);
+    if (sp->M) (void)hipFree(sp->M);
     delete sp;
 }
 
@@ -146,8 +173,21 @@ int spectral_apply(SpectralPlan* sp, hipStream_t s, const double* in, double* ou
     SP_FFT(hipfftExecD2Z(sp->fwd, const_cast<double*>(in), (hipfftDoubleComplex*)sp->X));
     const long long total = (long long)sp->nk * sp->nj * (sp->ni / 2 + 1);
     const double scale = 1.0 / ((double)sp->ni * sp->nj * sp->nk);
-    hipLaunchKernelGGL(k_spectral_rk4, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, sp->X, sp->ni, sp->nj,
-                       sp->nk, st, scale);
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (st.nu_tab) {
+        hipLaunchKernelGGL(k_spectral_rk4, grid, dim3(256), 0, s, sp->X, (double2*)nullptr, sp->ni, sp->nj, sp->nk, st,
+                           scale);
+    } else {
+        // constant nu: every segment of the same length has the same multiplier
+        if (!sp->m_valid || !same_segment(sp->m_key, st)) {
+            hipLaunchKernelGGL(k_spectral_rk4, grid, dim3(256), 0, s, (double2*)nullptr, sp->M, sp->ni, sp->nj, sp->nk,
+                               st, scale);
+            SP_HIP(hipGetLastError());
+            sp->m_key = st;
+            sp->m_valid = true;
+        }
+        hipLaunchKernelGGL(k_spectral_apply, grid, dim3(256), 0, s, sp->X, (const double2*)sp->M, total);
+    }
     SP_HIP(hipGetLastError());
     SP_FFT(hipfftExecZ2D(sp->inv, (hipfftDoubleComplex*)sp->X, out));
     return SP_OK;
