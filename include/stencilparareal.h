@@ -45,6 +45,8 @@ extern "C" {
 #define SP_COARSE_RK4_SPECTRAL 2 /* the same RK4 map, each coarse segment applied as its exact
                                     Fourier multiplier (FFT, product, inverse FFT; FP64 only;
                                     no counterpart in the reference) */
+#define SP_COARSE_EULER_SPECTRAL 3 /* the DoEuler map (upwind per sp_prop_set_upwind) applied
+                                      the same way (FP64 only; no counterpart in the reference) */
 
 /* ---- kernel paths for the fine propagator ------------------------------------- */
 #define SP_PATH_FUSED 0  /* one kernel per RK4 step, 4 stages fused (default) */
@@ -219,7 +221,8 @@ typedef struct sp_parareal_cfg {
     double end_time;     /* T */
     int steps_fine;      /* total fine steps over [0, T] */
     int steps_coarse;    /* total coarse steps over [0, T] */
-    int coarse_kind;     /* SP_COARSE_EULER (reference), SP_COARSE_RK4 or SP_COARSE_RK4_SPECTRAL */
+    int coarse_kind;     /* SP_COARSE_EULER (reference), SP_COARSE_RK4, SP_COARSE_RK4_SPECTRAL or
+                            SP_COARSE_EULER_SPECTRAL */
     int coarse_dtype;    /* SP_F64, SP_F32 or SP_BF16 (RK4 only for the latter two) */
     int coarse_substeps; /* >= 1: each coarse step is this many sub-steps */
     double tol;          /* stop once the relative L2 increment <= tol; 0: run kmax */

@@ -19,7 +19,7 @@ SP_ERR_DIVERGED = 4
 SP_ERR_NOMEM = 5
 
 SP_F64, SP_F32, SP_BF16 = 0, 1, 2
-SP_COARSE_EULER, SP_COARSE_RK4, SP_COARSE_RK4_SPECTRAL = 0, 1, 2
+SP_COARSE_EULER, SP_COARSE_RK4, SP_COARSE_RK4_SPECTRAL, SP_COARSE_EULER_SPECTRAL = 0, 1, 2, 3
 SP_PATH_FUSED, SP_PATH_STAGED = 0, 1
 SP_KERNEL_AUTO, SP_KERNEL_SCALAR, SP_KERNEL_PAIR, SP_KERNEL_QUAD, SP_KERNEL_SPLIT = -1, 0, 1, 2, 3
 SP_MAX_ITERS = 64

@@ -895,7 +895,7 @@ struct sp_parareal {
     Buf U, F, FS, Gold, Gw0, Gw1, Q, coefF, coefG, part;
     Buf D;  // difference form: U_n^{k+1} - U_n^k of the chain's current slice
     Buf W;  // work field of split-tile steps (its own: captured graphs keep its address)
-    SpectralPlan* spec = nullptr;  // SP_COARSE_RK4_SPECTRAL: FFT plans and coefficient buffer
+    SpectralPlan* spec = nullptr;  // spectral coarse kinds: FFT plans and coefficient buffer
     Buf nu_spec;  // ... and its per-sub-step nu: [0, S*tsG) from t = 0, then [i*S + r] per slice
     std::vector<double> nuF_host, nuG_host;
     bool nu_const = true;
@@ -935,6 +935,9 @@ FusedStep pr_step(const sp_parareal* pr, int nslices, double dt, const void* nud
     return st;
 }
 
+// The coarse kinds that apply each segment as its Fourier multiplier (sp_spectral.hip).
+bool spectral_coarse(int kind) { return kind == SP_COARSE_RK4_SPECTRAL || kind == SP_COARSE_EULER_SPECTRAL; }
+
 // Coarse propagation of `nslices` batched slices, tsG sub-steps from per-slice start
 // times; step i of slice r runs at t0[r] + i*dtG (or at (first + i)*dtG from t = 0 for
 // the initial chain, first_index >= 0).  Input f64 (U) or gdt; output gdt in `dst`.
@@ -944,7 +947,7 @@ int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View
     sp_prop* p = pr->p;
     hipStream_t s = p->ctx->stream;
     const int gdt = odt < 0 ? pr->gdt : odt;  // the chain's storage type (FP64 for an FP64 initial run)
-    if (pr->cfg.coarse_kind == SP_COARSE_RK4_SPECTRAL) {
+    if (spectral_coarse(pr->cfg.coarse_kind)) {
         // each slice's segment as one Fourier multiplier (compact FP64 fields)
         SP_CHECK(idt == SP_F64 && gdt == SP_F64 && src.sj == p->ni && dst.sj == p->ni &&
                      src.sk == (long long)p->ni * p->nj && dst.sk == (long long)p->ni * p->nj,
@@ -952,6 +955,8 @@ int coarse_run(sp_parareal* pr, int nslices, int slice0, int idt, View src, View
         const long long per = (long long)pr->S * pr->tsG;
         for (int q = 0; q < nslices; ++q) {
             SpectralStep st{p->dx, p->cx, p->cy, p->cz, pr->dtG, steps, p->nu0, nullptr, 1};
+            if (pr->cfg.coarse_kind == SP_COARSE_EULER_SPECTRAL)
+                st.kind = p->upwind ? kSpectralEulerUpwind : kSpectralEulerCentral;
             if (!pr->nu_const) {
                 const double* tab = (const double*)pr->nu_spec.p;
                 if (first_index >= 0) {
@@ -1127,9 +1132,9 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
     SP_CHECK(cfg->end_time > 0 && std::isfinite(cfg->end_time), SP_ERR_INVALID, "sp_parareal_create: end_time");
     SP_CHECK(cfg->steps_fine >= 1 && cfg->steps_coarse >= 1, SP_ERR_INVALID, "sp_parareal_create: step counts");
     SP_CHECK(cfg->coarse_kind == SP_COARSE_EULER || cfg->coarse_kind == SP_COARSE_RK4 ||
-                 cfg->coarse_kind == SP_COARSE_RK4_SPECTRAL,
+                 spectral_coarse(cfg->coarse_kind),
              SP_ERR_INVALID, "sp_parareal_create: coarse_kind");
-    SP_CHECK(cfg->coarse_kind != SP_COARSE_RK4_SPECTRAL || cfg->coarse_dtype == SP_F64, SP_ERR_UNSUPPORTED,
+    SP_CHECK(!spectral_coarse(cfg->coarse_kind) || cfg->coarse_dtype == SP_F64, SP_ERR_UNSUPPORTED,
              "sp_parareal_create: the spectral coarse propagator runs in FP64");
     SP_CHECK(cfg->coarse_dtype == SP_F64 || cfg->coarse_dtype == SP_F32 || cfg->coarse_dtype == SP_BF16,
              SP_ERR_INVALID, "sp_parareal_create: coarse_dtype");
@@ -1206,7 +1211,7 @@ int sp_parareal_create(sp_prop* p, const sp_parareal_cfg* cfg, sp_parareal** out
             return r;
         }
     }
-    if (cfg->coarse_kind == SP_COARSE_RK4_SPECTRAL) {
+    if (spectral_coarse(cfg->coarse_kind)) {
         r = spectral_create(&pr->spec, p->ni, p->nj, p->nk);
         if (r == SP_OK && !pr->nu_const) {
             // nu of every coarse sub-step: from t = 0 (the initial run), then per slice start
@@ -1451,7 +1456,7 @@ static int iterate_impl(sp_parareal* pr, double* incr, double* tfine, double* tc
         }
         // The spectral coarse chain (S FFT pairs) is launched, not captured: its FFT library
         // calls stay out of the graph.
-        const bool chain_graph = pr->cfg.coarse_kind != SP_COARSE_RK4_SPECTRAL;
+        const bool chain_graph = !spectral_coarse(pr->cfg.coarse_kind);
         if (!pr->gexecF) {
             SP_TRY(capture(pr, s, &pr->graphF, &pr->gexecF, [&] { return fine_sweep(pr); }));
             if (chain_graph)

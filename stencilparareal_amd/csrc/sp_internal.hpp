@@ -148,9 +148,14 @@ struct FusedStep {
     void* work = nullptr;  // SP_KERNEL_SPLIT: FP64 field of nslices compact slices (v2)
     size_t work_bytes = 0;
 };
-// The RK4 coarse propagator as a Fourier multiplier (sp_spectral.hip): one segment of
-// `steps` sub-steps of length h on a compact FP64 field, nu per sub-step from a device table
-// (nu_tab[s * nu_stride]) or constant (nu_tab == nullptr).
+// A coarse propagator as a Fourier multiplier (sp_spectral.hip): one segment of `steps`
+// sub-steps of length h on a compact FP64 field, nu per sub-step from a device table
+// (nu_tab[s * nu_stride]) or constant (nu_tab == nullptr).  kind: the sub-step's map.
+enum SpectralKind : int {
+    kSpectralRK4 = 0,           // DoRK4Timestep: RK4 of Advection4 + nu Laplace4
+    kSpectralEulerUpwind = 1,   // DoEulerTimestep, first-order upwind Advection2 + nu Laplace2
+    kSpectralEulerCentral = 2,  // DoEulerTimestep, central Advection2 + nu Laplace2
+};
 struct SpectralStep {
     double dx, cx, cy, cz;
     double h;
@@ -158,6 +163,7 @@ struct SpectralStep {
     double nu;
     const double* nu_tab;
     long long nu_stride;
+    int kind = kSpectralRK4;
 };
 struct SpectralPlan;
 int spectral_create(SpectralPlan** out, int ni, int nj, int nk);

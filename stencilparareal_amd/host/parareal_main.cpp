@@ -1,7 +1,8 @@
 // parareal -- single-GPU driver with the run modes and output of the reference's
 // src/Parareal/main.cpp (Compare, Serial, Parallel, Timing), built on the drop-in C++ API
 // (include/sp/*.hpp) over libsp_hip.so.  The slices the reference spreads over MPI ranks
-// are the batch index of one device (--timeSlices).  Extensions: --coarse euler|rk4|rk4spectral,
+// are the batch index of one device (--timeSlices).  Extensions:
+// --coarse euler|rk4|rk4spectral|eulerspectral,
 // --coarseDtype f64|f32|bf16, --initialFP64 0|1 (f32 / bf16 G: the initial coarse run in FP64),
 // --tol <t>, --graphs 0|1, --autotune 0|1.  Energy counters of the Cray nodes are not
 // available and not printed.
@@ -60,7 +61,10 @@ int main(int argc, char** argv)
         const std::string k = argv[a];
         const std::string v = a + 1 < argc ? argv[a + 1] : "";
         if (k == "--coarse")
-            opt.coarse_kind = v == "rk4" ? SP_COARSE_RK4 : v == "rk4spectral" ? SP_COARSE_RK4_SPECTRAL : SP_COARSE_EULER;
+            opt.coarse_kind = v == "rk4"             ? SP_COARSE_RK4
+                              : v == "rk4spectral"   ? SP_COARSE_RK4_SPECTRAL
+                              : v == "eulerspectral" ? SP_COARSE_EULER_SPECTRAL
+                                                     : SP_COARSE_EULER;
         if (k == "--coarseDtype") opt.coarse_dtype = v == "f32" ? SP_F32 : v == "bf16" ? SP_BF16 : SP_F64;
         if (k == "--tol") opt.tol = std::atof(v.c_str());
         if (k == "--graphs") opt.use_graphs = v != "0";

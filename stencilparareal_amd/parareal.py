@@ -19,9 +19,9 @@ from .field import ConvectionField
 class Parareal:
     """Parareal(propagator, initial, solution, conf) (Parareal.h:48-108).
 
-    coarse: "euler" (the reference's DoEuler coarse propagator), "rk4", or "rk4_spectral" (the
-    same RK4 coarse map applied per segment as its exact Fourier multiplier: FFT, product,
-    inverse FFT; FP64).
+    coarse: "euler" (the reference's DoEuler coarse propagator), "rk4", "rk4_spectral" or
+    "euler_spectral" (the same RK4 / Euler coarse map applied per segment as its exact Fourier
+    multiplier: FFT, product, inverse FFT; FP64).
     coarse_dtype: "f64", "f32" or "bf16" (storage; f32 arithmetic) for the RK4 coarse.
     tol: stop once the relative L2 increment of the slice states is <= tol (0 = run
     exactly kmax iterations, as the reference does, Parareal.h:116).
@@ -48,7 +48,8 @@ class Parareal:
         cfg.steps_fine = conf.timeStepsFine
         cfg.steps_coarse = conf.timeStepsCoarse
         cfg.coarse_kind = {"euler": L.SP_COARSE_EULER, "rk4": L.SP_COARSE_RK4,
-                           "rk4_spectral": L.SP_COARSE_RK4_SPECTRAL}[coarse]
+                           "rk4_spectral": L.SP_COARSE_RK4_SPECTRAL,
+                           "euler_spectral": L.SP_COARSE_EULER_SPECTRAL}[coarse]
         cfg.coarse_dtype = L.DTYPE_CODES[coarse_dtype]
         cfg.coarse_substeps = substeps
         cfg.tol = tol

@@ -153,3 +153,13 @@ def test_driver_parallel_serial_timing_modes():
     times = [float(x) for x in out.split("Times:")[1].split()]
     assert len(times) == 5 * ARGS["kmax"] + 2
     assert all(b >= a for a, b in zip(times, times[1:])), times
+
+
+@pytest.mark.gpu
+def test_driver_euler_spectral_coarse_matches_oracle():
+    # --coarse eulerspectral: the reference's Euler coarse propagator applied per segment as
+    # its Fourier multiplier; the Parareal error is the stencil Euler one's (the oracle's)
+    out = run(argv("parallel", nufreq=20.0) + ["--coarse", "eulerspectral"])
+    got = field(out, "Error of parareal")
+    want = oracle_error(nufreq=20.0)
+    assert abs(got - want) <= 2e-4 * want, (got, want)  # printed as %.4e

@@ -361,13 +361,14 @@ def test_invalid_arguments_return_errors(sp):
 # ---------------------------------------------------------------------------------------------
 # Parareal
 
-def _parareal_case(sp, path, coarse, N=12, S=4, kmax=4, nufreq=0.0, async_=False):
+def _parareal_case(sp, path, coarse, N=12, S=4, kmax=4, nufreq=0.0, async_=False, upwind=True):
     conf = RuntimeConfiguration(gridSize=N, endTime=0.02, timeStepsFine=64, timeStepsCoarse=8, nu0=0.05,
                                 nufreq=nufreq, cx=1.0, cy=-0.5, cz=0.25, kmax=kmax, async_=async_,
                                 timeSlices=S)
-    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, path=path)
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, path=path,
+                         upwind=upwind)
     u0 = O.fill_gaussians(N, [0.3, 0.6], [0.4, 0.7], [0.5, 0.2], [0.1, 0.08], [1.0, 0.5])
-    oc = O.OConvection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz)
+    oc = O.OConvection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, upwind=upwind)
     _, iters = oc.parareal(u0, S, kmax, conf.endTime, conf.dtFine(), conf.timeStepsFinePerTimeSlice(),
                            conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(), coarse, async_=async_)
     serial = oc.serial(u0, conf.dtFine(), conf.timeStepsFine)
@@ -527,6 +528,32 @@ def test_parareal_spectral_coarse_vs_oracle(sp, graphs, nufreq):
         assert relerr(pr.solution().get(), iters[k]) <= 1e-12, k
     with pytest.raises(Exception):
         sp.Parareal(conv, init, sol, conf, coarse="rk4_spectral", coarse_dtype="f32")
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("upwind", [True, False])
+@pytest.mark.parametrize("nufreq", [0.0, 80.0])
+def test_parareal_euler_spectral_coarse_vs_oracle(sp, graphs, upwind, nufreq):
+    # the reference's Euler coarse propagator (Advection2 upwind or central + Laplace2)
+    # applied per segment as its Fourier multiplier: the prediction is the oracle's stencil
+    # Euler chain to rounding, and every iterate the oracle's within 1e-12
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "euler", N=16, nufreq=nufreq, upwind=upwind)
+    init = conv.field("u0")
+    init.set(u0)
+    sol = conv.field("sol")
+    pr = sp.Parareal(conv, init, sol, conf, coarse="euler_spectral", use_graphs=graphs)
+    pr.begin()
+    assert relerr(pr.solution().get(), iters[0]) <= 1e-13
+    for k in range(1, conf.kmax + 1):
+        pr.iterate()
+        assert relerr(pr.solution().get(), iters[k]) <= 1e-12, k
+    # the other branch gives another prediction: the kind follows the propagator's setting
+    oc = O.OConvection(16, 16, 16, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, upwind=not upwind)
+    _, other = oc.parareal(u0, conf.timeSlices, 0, conf.endTime, conf.dtFine(), conf.timeStepsFinePerTimeSlice(),
+                           conf.dtCoarse(), conf.timeStepsCoarsePerTimeSlice(), "euler")
+    assert relerr(other[0], iters[0]) > 1e-6
+    with pytest.raises(Exception):
+        sp.Parareal(conv, init, sol, conf, coarse="euler_spectral", coarse_dtype="f32")
 
 
 def test_parareal_autotuned_vs_oracle(sp):
