@@ -27,7 +27,7 @@ Timed once beside the steps (not part of `value`):
     32^3 / 4-slice CPU configuration's Parareal in full.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-solve] [--no-cpu]
-                    [--workload cfg4|cfg2|cfg5] [--roofline-only]
+                    [--workload cfg4|cfg2|cfg5|ref] [--roofline-only]
 """
 from __future__ import annotations
 
@@ -485,6 +485,78 @@ def cfg5(a):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------------------
+# the reference's own speedup job on one GPU
+
+def refjob(a):
+    """The reference's GPU speedup job (jobs/paperJobs/04-SpeedupGPU.job:42-66) on one GPU:
+    128^3, nu0 = 0.1, c = (1,1,1), T = 0.1, sin-product initial field, F = 32768 RK4 steps,
+    G = 2048 forward-Euler steps with first-order upwind advection, K = 3, blocking schedule,
+    4 ... 128 slices batched.  Each slice count runs with the stencil Euler G and with the
+    same G applied per segment as its Fourier multiplier, beside the reference's published
+    wall time (one GPU per slice) and K = 3 error (tests/golden/published.json, from the
+    job's run_<N>_0.log).  value = fine lattice updates per second of the 128-slice solve
+    with the Fourier-multiplier G."""
+    import torch
+    import stencilparareal_amd as sp
+    from stencilparareal_amd.config import RuntimeConfiguration
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    with open(os.path.join(ROOT, "tests", "golden", "published.json")) as f:
+        pub = json.load(f)
+    st, runs = pub["setup"], pub["runs"]["04-SpeedupGPU"]
+    N, K = st["gridSize"], 3
+
+    def conf_for(S):
+        return RuntimeConfiguration(gridSize=N, endTime=st["endTime"], timeStepsFine=st["timeStepsFine"],
+                                    timeStepsCoarse=st["timeStepsCoarse"], nu0=st["nu0"], nufreq=0.0, cx=st["cx"],
+                                    cy=st["cy"], cz=st["cz"], kmax=K, async_=False, timeSlices=S)
+
+    c4 = conf_for(4)
+    conv = sp.Convection(N, N, N, c4.dx(), c4.nu0, 0.0, c4.cx, c4.cy, c4.cz, upwind=True)
+    init, ref, q = conv.field("qinitial"), conv.field("qreference"), conv.field("q")
+    sp.fill_exact(init, c4.nu0, 0.0, c4.cx, c4.cy, c4.cz, 0.0)
+
+    def timed(fn):
+        fn()  # first call: autotuning memo, graph capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    pr = sp.Parareal(conv, init, ref, c4, coarse="euler", use_graphs=True, autotune=True)
+    log("serial fine solve (32768 steps)")
+    t_serial, _ = timed(pr.DoSerial)
+    del pr
+    rows = []
+    for S in (4, 8, 16, 32, 64, 128):
+        pubS = runs[str(S)]
+        row = {"slices": S, "published_wall_s": pubS["Parallel run time"], "published_error": pubS["Error at end"]}
+        for g in ("euler", "euler_spectral"):
+            log(f"{S} slices, coarse {g}")
+            pr = sp.Parareal(conv, init, q, conf_for(S), coarse=g, use_graphs=True, autotune=True)
+            t, rep = timed(pr.DoParallel)
+            err = sp.error_reference(q, ref)
+            row[g] = {"wall_s": round(t, 4), "error": float(f"{err:.6e}"),
+                      "fine_s": round(sum(rep.t_fine[:K]), 4), "chain_s": round(sum(rep.t_coarse[:K]), 4),
+                      "vs_published_wall": round(pubS["Parallel run time"] / t, 2)}
+            del pr
+        rows.append(row)
+    last = rows[-1]["euler_spectral"]["wall_s"]
+    fine_lup = K * st["timeStepsFine"] * N ** 3
+    out = {"metric": METRIC, "value": round(fine_lup / last / 1e9, 3), "unit": "GLUP/s", "n_gpus": 1,
+           "steps": 1, "warmup": 1, "ms_per_step": round(last * 1e3, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "the reference job's own initial field (sin product, fillQ at t = 0)",
+           "config": {"workload": "reference speedup job: 128^3, nu0=0.1, c=(1,1,1), T=0.1, F 32768 RK4 steps, "
+                                  "G 2048 Euler upwind steps, K=3, 4..128 slices batched on one GPU",
+                      "grid": N, "kmax": K, "parallelism": "slices-as-batch",
+                      "step": "one K=3 Parareal solve over 128 slices (Fourier-multiplier Euler G)"},
+           "serial_fine": {"wall_s": round(t_serial, 4), "published_wall_s": runs["4"]["Serial run time"]},
+           "by_slices": rows, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def roofline_only(a):
     """Only the dominant kernel and a copy kernel of the same bytes (the PMC scale), for
     profilers: a.reps batched steps of the workload's slices."""
@@ -531,7 +603,7 @@ def main():
                     help="cfg4: G variants solved beside the configured BF16 one (comma list of f64, f32, bf16; "
                          "':init64' for the initial coarse run in FP64, 'f64:spectral' for the FP64 RK4 coarse "
                          "map as a Fourier multiplier; '' for none)")
-    ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5"], default="cfg4",
+    ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5", "ref"], default="cfg4",
                     help="cfg4: 384^3 x 64 slices, BF16 G (default); cfg2: 128^3 x 16 slices, FP64; "
                          "cfg5: the 24-problem uneven batch")
     ap.add_argument("--roofline-only", action="store_true",
@@ -553,7 +625,7 @@ def main():
         sys.exit(2)
     if a.roofline_only:
         return roofline_only(a)
-    return {"cfg4": cfg4, "cfg2": cfg2, "cfg5": cfg5}[a.workload](a)
+    return {"cfg4": cfg4, "cfg2": cfg2, "cfg5": cfg5, "ref": refjob}[a.workload](a)
 
 
 if __name__ == "__main__":
