@@ -1262,34 +1262,42 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
             qv3[f] = ld2(rin, q3[f], z3);
         }
     }
-    d2 pre[M0], pre2[M0];  // q planes t+1 and t+2 in flight
-    if (tq0 + 1 < tq1) {
-        const int zo = wrapz(tq0 + 1, nk) * iskb;
+    // q planes t+1 and t+2 in flight, in two register sets by plane parity: the set plane
+    // t writes to P0 is refilled with plane t+3 at once, so each load has two planes to land
+    // and no register copy waits on it.  (Set (R+1)&1 at phase R: plane tq0+1 in set 1,
+    // plane tq0+2 in set 0.)
+    // (loaded whatever tq1 is, like the planes' loads below: every path into the plane
+    // loop issues the same loads)
+    d2 preA[M0], preB[M0];
+    {
+        const int zo1 = wrapz(tq0 + 1, nk) * iskb, zo2 = wrapz(tq0 + 2, nk) * iskb;
 #pragma unroll
-        for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
-    }
-    if (tq0 + 2 < tq1) {
-        const int zo = wrapz(tq0 + 2, nk) * iskb;
+        for (int m = 0; m < M0; ++m) preB[m] = ld2(rin, off0[m], zo1);
 #pragma unroll
-        for (int m = 0; m < M0; ++m) pre2[m] = ld2(rin, off0[m], zo);
+        for (int m = 0; m < M0; ++m) preA[m] = ld2(rin, off0[m], zo2);
     }
     __syncthreads();
 
+    // One plane at phase R = (t - tq0) & 3.  Every plane issues the same global loads at the
+    // same points, whichever stages are active (a stage's q load fetches the plane it needs
+    // at the next plane, so loads ahead of its first active plane fetch the same values;
+    // past tq1 they fetch valid wrapped planes that no stage reads): the compiler's memory
+    // waits then count them exactly instead of waiting for every load and store of the
+    // previous plane.  Only the output store depends on the plane.
     auto plane = [&](auto Rc, int t) {
         constexpr int R = decltype(Rc)::value;
+        d2(&pre)[M0] = (R & 1) ? preA : preB;  // plane t+1, then t+3
         const int cur = t & 1, prv = cur ^ 1;
         __builtin_amdgcn_sched_barrier(0);
         if (t + 1 < tq1) {
 #pragma unroll
             for (int m = 0; m < M0; ++m)
                 if (wr0(m)) *(d2*)&P0[prv][dst0[m]] = pre[m];
+        }
+        {
+            const int zo = wrapz(t + 3, nk) * iskb;
 #pragma unroll
-            for (int m = 0; m < M0; ++m) pre[m] = pre2[m];
-            if (t + 3 < tq1) {
-                const int zo = wrapz(t + 3, nk) * iskb;
-#pragma unroll
-                for (int m = 0; m < M0; ++m) pre2[m] = ld2(rin, off0[m], zo);
-            }
+            for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
         }
         __builtin_amdgcn_sched_barrier(0);
         if (t < tq1) {
@@ -1313,7 +1321,10 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
                     *(d2*)&P2[cur][o2 + f * W2] =
                         d2{fmaT(w.f2, d2v[2 * f], qv2[f].x), fmaT(w.f2, d2v[2 * f + 1], qv2[f].y)};
             }
-            const int zo = wrapz(t - 4, nk) * iskb;  // q(t-4): the next plane's q
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        {
+            const int zo = wrapz(t - 4, nk) * iskb;  // q(t-4): stage 2's q at the next plane
 #pragma unroll
             for (int f = 0; f < 2; ++f) qv2[f] = ld2(rin, q2[f], zo);
             __builtin_amdgcn_sched_barrier(0);
@@ -1328,7 +1339,10 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
                     *(d2*)&Q4[cur][wq + f * WQ] = qv3[f];  // q(t-8) for stage 4
                 }
             }
-            const int zo = wrapz(t - 7, nk) * iskb;  // q(t-7): the next plane's q
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        {
+            const int zo = wrapz(t - 7, nk) * iskb;  // q(t-7): stage 3's q at the next plane
 #pragma unroll
             for (int f = 0; f < 2; ++f) qv3[f] = ld2_last(rin, q3[f], zo);
             __builtin_amdgcn_sched_barrier(0);
@@ -1345,7 +1359,6 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
     };
-
     for (int t = tq0; t < tend; t += 4) {
         plane(std::integral_constant<int, 0>{}, t);
         if (t + 1 < tend) plane(std::integral_constant<int, 1>{}, t + 1);
