@@ -442,6 +442,12 @@ __device__ __forceinline__ void st2_stream(__amdgpu_buffer_rsrc_t r, int vo, int
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, vo, so, 2);
 }
+// 16 bytes per lane from global memory straight into LDS: lane l of the wave writes
+// lds + 16 l (lds: the wave's base, the same for every lane).
+__device__ __forceinline__ void ld2_to_lds(__amdgpu_buffer_rsrc_t r, double* lds, int vo, int so)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, vo, so, 0, 0);
+}
 
 // Two x-adjacent values of compute type T (d2 for FP64), and their global access for
 // storage type S: 2 * sizeof(S) bytes per lane (FP64 16, FP32 8, BF16 4), converted to T.
@@ -1172,10 +1178,13 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
     constexpr int H0 = G1::HI;                         // rows of the q plane incl. halo
     constexpr int SLOTS0 = QuadCfg::VW0 * H0;          // loader slots (cols >= WI/2 fill padding)
     constexpr int M0 = (SLOTS0 + NT - 1) / NT;
-    static_assert(QuadCfg::VW0 * 2 <= QuadCfg::W0 && (QuadCfg::W0 / 2 - QuadCfg::VW0) % 16 == 0, "loader layout");
+    static_assert(QuadCfg::VW0 * 2 == QuadCfg::W0, "loader slots fill the q plane's rows contiguously");
     constexpr int W0 = QuadCfg::W0, W1 = QuadCfg::W1, W2 = QuadCfg::W2, W3 = QuadCfg::W3, WQ = QuadCfg::WQ;
 
-    __shared__ __attribute__((aligned(16))) double P0[2][H0 * W0];
+    // q planes of even and odd phase (chunks start on even planes): two arrays, so the
+    // compiler sees that stage 1's reads of one never alias the load into the other
+    __shared__ __attribute__((aligned(16))) double P0a[H0 * W0];
+    __shared__ __attribute__((aligned(16))) double P0b[H0 * W0];
     __shared__ __attribute__((aligned(16))) double P1[2][Geo<TX, TY, 2>::HI * W1];
     __shared__ __attribute__((aligned(16))) double P2[2][Geo<TX, TY, 3>::HI * W2];
     __shared__ __attribute__((aligned(16))) double P3[2][G4::HI * W3];
@@ -1197,6 +1206,9 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
 
     // q plane loader: VW0 lanes per row; lanes past the row's 24 pairs load the periodic
     // continuation into the row's padding (never read), which keeps the writes conflict-free
+    // slot g of the loader is P0 element 2 g: wave w's m-th load fills slots
+    // m NT + 64 w .. + 63, one contiguous kilobyte
+    const int wave = tid >> 6;
     int off0[M0], dst0[M0];
 #pragma unroll
     for (int m = 0; m < M0; ++m) {
@@ -1254,27 +1266,13 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
         const int zo = wrapz(tq0, nk) * iskb;
 #pragma unroll
         for (int m = 0; m < M0; ++m)
-            if (wr0(m)) *(d2*)&P0[tq0 & 1][dst0[m]] = ld2(rin, off0[m], zo);
+            if (wr0(m)) *(d2*)&P0a[dst0[m]] = ld2(rin, off0[m], zo);
         const int z2 = wrapz(z0 - 8, nk) * iskb, z3 = wrapz(z0 - 6, nk) * iskb;
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             qv2[f] = ld2(rin, q2[f], z2);
             qv3[f] = ld2(rin, q3[f], z3);
         }
-    }
-    // q planes t+1 and t+2 in flight, in two register sets by plane parity: the set plane
-    // t writes to P0 is refilled with plane t+3 at once, so each load has two planes to land
-    // and no register copy waits on it.  (Set (R+1)&1 at phase R: plane tq0+1 in set 1,
-    // plane tq0+2 in set 0.)
-    // (loaded whatever tq1 is, like the planes' loads below: every path into the plane
-    // loop issues the same loads)
-    d2 preA[M0], preB[M0];
-    {
-        const int zo1 = wrapz(tq0 + 1, nk) * iskb, zo2 = wrapz(tq0 + 2, nk) * iskb;
-#pragma unroll
-        for (int m = 0; m < M0; ++m) preB[m] = ld2(rin, off0[m], zo1);
-#pragma unroll
-        for (int m = 0; m < M0; ++m) preA[m] = ld2(rin, off0[m], zo2);
     }
     __syncthreads();
 
@@ -1286,23 +1284,22 @@ __device__ __forceinline__ void rk4_quad_body(const FusedArgs& a, const int bid)
     // previous plane.  Only the output store depends on the plane.
     auto plane = [&](auto Rc, int t) {
         constexpr int R = decltype(Rc)::value;
-        d2(&pre)[M0] = (R & 1) ? preA : preB;  // plane t+1, then t+3
         const int cur = t & 1, prv = cur ^ 1;
+        double* const P0cur = (R & 1) ? P0b : P0a;  // q(t) (t - tq0 = R mod 4, tq0 even)
+        double* const P0nxt = (R & 1) ? P0a : P0b;
         __builtin_amdgcn_sched_barrier(0);
-        if (t + 1 < tq1) {
+        {
+            // q plane t+1 into P0nxt (read by stage 1 at the next plane; the wait for it
+            // comes before this plane's barrier)
+            const int zo = wrapz(t + 1, nk) * iskb;
 #pragma unroll
             for (int m = 0; m < M0; ++m)
-                if (wr0(m)) *(d2*)&P0[prv][dst0[m]] = pre[m];
-        }
-        {
-            const int zo = wrapz(t + 3, nk) * iskb;
-#pragma unroll
-            for (int m = 0; m < M0; ++m) pre[m] = ld2(rin, off0[m], zo);
+                if (wr0(m)) ld2_to_lds(rin, &P0nxt[2 * (m * NT + wave * 64)], off0[m], zo);
         }
         __builtin_amdgcn_sched_barrier(0);
         if (t < tq1) {
             // centres of plane t are q(t); r1b = q(t-2) is the update's q
-            quad_step<1, R & 3>(P0[cur], w, acc1, d1, c1);
+            quad_step<1, R & 3>(P0cur, w, acc1, d1, c1);
 #pragma unroll
             for (int f = 0; f < 2; ++f) {
                 if (live1)
@@ -1475,6 +1472,9 @@ int fill_args(FusedArgs& a, const FusedStep& st, const FusedTile& tile, View in,
     a.nj = st.nj;
     a.nk = st.nk;
     a.kchunk = tile.kchunk > 0 ? min(tile.kchunk, st.nk) : st.nk;
+    // the 2x2 kernel's chunks start on even planes (its q-plane buffers alternate by plane
+    // parity); the chunking does not change the results
+    if (tile.variant == SP_KERNEL_QUAD && a.kchunk < st.nk && (a.kchunk & 1)) a.kchunk += 1;
     a.tiles_x = (st.ni + tile.tx - 1) / tile.tx;
     const int tiles_y = (st.nj + tile.ty - 1) / tile.ty;
     a.tiles = a.tiles_x * tiles_y;
