@@ -27,7 +27,7 @@ Timed once beside the steps (not part of `value`):
     32^3 / 4-slice CPU configuration's Parareal in full.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-solve] [--no-cpu]
-                    [--workload cfg4|cfg2|cfg5|ref] [--roofline-only]
+                    [--workload cfg4|cfg3|cfg2|cfg5|ref] [--roofline-only]
 """
 from __future__ import annotations
 
@@ -339,6 +339,125 @@ def cfg4_solve(a, w, conf, conv, init):
 
 
 # ---------------------------------------------------------------------------------------
+# the 256^3 / 32-slice FP32-G workload (configuration 3)
+
+def per_iterate_solve(conv, init, conf, w, sol, serial, gdt, tol=1e-6):
+    """Parareal with G in storage type `gdt`, kmax iterations, the error of every iterate
+    (the last slice's state, as DoParallel returns it) against the serial fine solution;
+    the elapsed solve time counts begin and the iterations only."""
+    import torch
+    import stencilparareal_amd as sp
+    log(f"Parareal, G {gdt}, K = {w.kmax}: error of every iterate")
+    pr = sp.Parareal(conv, init, sol, conf, coarse="rk4", coarse_dtype=gdt, use_graphs=True, autotune=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr.begin()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    pr.solution(sol)
+    rows = [{"k": 0, "rel_l2_err": float(f"{sp.rel_l2(sol, serial):.3e}"),
+             "rel_linf_err": float(f"{sp.error_reference(sol, serial):.3e}"), "elapsed_s": round(t, 3)}]
+    first = None
+    for k in range(1, w.kmax + 1):
+        t0 = time.perf_counter()
+        inc = pr.iterate()
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+        f_s, c_s, _ = pr.iteration_times()
+        pr.solution(sol)
+        e = sp.rel_l2(sol, serial)
+        rows.append({"k": k, "increment": float(f"{inc:.3e}"), "rel_l2_err": float(f"{e:.3e}"),
+                     "rel_linf_err": float(f"{sp.error_reference(sol, serial):.3e}"),
+                     "fine_s": round(f_s, 4), "chain_s": round(c_s, 4), "elapsed_s": round(t, 3)})
+        log(f"  iterate {k}: increment {inc:.3e}, rel. L2 error {e:.3e}, fine sweep {f_s:.2f} s, chain {c_s:.3f} s")
+        if first is None and e <= tol:
+            first = {"iterate": k, "wall_s": round(t, 3)}
+    del pr
+    return {"g_dtype": gdt, "iterations": w.kmax, "wall_s": round(t, 3), "per_iterate": rows,
+            f"first_within_{tol:g}": first}
+
+
+def cfg3(a):
+    """Configuration 3: 256^3 grid, 32 slices batched on one GPU, 8 seeded Gaussians, c
+    uniform in [-1, 1]^3 (seed 1), nu = 2e-3, T = 1; F FP64 RK4 dt = 2.5e-5 (1250 steps per
+    slice), G RK4 in FP32 with dt = 1e-3 (difference-form correction), K = 5.  value = the
+    batched fine step's lattice updates per second; beside it, once: the serial FP64 fine
+    solve, the Parareal solve with G in FP32 (as configured) and in FP64 with the error of
+    every iterate against the serial solution, and plain all-FP64 Parareal in the reference's
+    structure, extrapolated from measured step times at the iterate count FP64 G needs for
+    1e-6."""
+    import torch
+    import stencilparareal_amd as sp
+    from stencilparareal_amd.config import workload
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    w = workload(3)
+    N, S = w.N, w.slices
+    conf = conf_of(w)
+    tsF, dtF = conf.timeStepsFinePerTimeSlice(), conf.dtFine()
+    conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
+    init = conv.field("qinitial")
+    g = w.gauss
+    sp.fill_gaussians(init, g.xc, g.yc, g.zc, g.sigma, g.amp)
+    log(f"autotune the fused tile for {S} slices")
+    conv.autotune(S)
+    tile = conv.get_tile(S, True)
+    bt = Batch2(conv, init, S)
+    log(f"warmup {a.warmup}, timed {a.steps} batched steps")
+    ms, k_ms = timed_steps(bt, a, dtF)
+    copy = bt.copy_gbs()
+    bt.free()
+    lup = S * N ** 3
+    kernel = (f"k_rk4_fused_{tile['variant']} {tile['tx']}x{tile['ty']} tiles, {tile['nthreads']} threads "
+              f"(FP64, {S} slices of {N}^3, kchunk {tile['kchunk']})")
+    rl = roofline(k_ms, 16.0 * lup, copy,
+                  committed_traffic(f"pmc_rk4_fused_{tile['variant']}_32x256cubed_fp64.txt"), kernel)
+    out = {
+        "metric": METRIC, "value": round(lup / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (8 seeded Gaussians, SplitMix64 seed 0; velocity seed 1)",
+        "config": {"workload": "256^3 grid, 32 time slices batched on one GPU, F FP64 RK4 dt=2.5e-5 (1250 steps "
+                               "per slice), G RK4 FP32 dt=1e-3, difference-form correction, K=5, T=1, nu=2e-3",
+                   "grid": N, "slices": S, "c": [round(x, 6) for x in w.c],
+                   "step": f"one batched fused RK4 fine step over all {S} slices ({S} x {N}^3 lattice updates)",
+                   "parallelism": "slices-as-batch"},
+        "roofline": rl,
+    }
+    if not a.no_solve:
+        sol, serial = conv.field("q"), conv.field("qreference")
+        log("autotune the one-slice FP64 tile (DoSerial); serial FP64 fine solve")
+        conv.autotune(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        conv.DoRK4(init, serial, 0.0, dtF, S * tsF)  # Parareal.h:250-253 (DoSerial)
+        torch.cuda.synchronize()
+        t_serial = time.perf_counter() - t0
+        runs = [per_iterate_solve(conv, init, conf, w, sol, serial, gdt) for gdt in (w.coarse_dtype, "f64")]
+        f64 = runs[1]["first_within_1e-06"]
+        K_plain = f64["iterate"] if f64 else w.kmax
+        log("plain comparator: reference-order step times")
+        tsG, dtG = conf.timeStepsCoarsePerTimeSlice(), conf.dtCoarse()
+        f_ms, g_ms = plain_step_ms(w, dtF), plain_step_ms(w, dtG)
+        plain_s = (K_plain * (S * tsF * f_ms + (S - 1) * tsG * g_ms) + S * tsG * g_ms) / 1e3
+        out.update({
+            "parareal": runs[0], "parareal_other_g": runs[1:],
+            "serial_fine": {"wall_s": round(t_serial, 3), "steps": S * tsF,
+                            "glups": round(S * tsF * N ** 3 / t_serial / 1e9, 3)},
+            "plain_fp64_parareal_reference_structure": {
+                "wall_s_extrapolated": round(plain_s, 1), "iterations": K_plain,
+                "iterations_from": "the FP64-G solve above (first iterate within 1e-6)",
+                "fine_step_ms": round(f_ms, 4), "coarse_step_ms": round(g_ms, 4)},
+            "to_1e-6_vs_serial_and_plain": {
+                r["g_dtype"]: ({"wall_s": r["first_within_1e-06"]["wall_s"],
+                                "speedup_vs_serial_fine": round(t_serial / r["first_within_1e-06"]["wall_s"], 3),
+                                "speedup_vs_plain_fp64_parareal": round(plain_s / r["first_within_1e-06"]["wall_s"], 2)}
+                               if r["first_within_1e-06"] else None) for r in runs},
+        })
+    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(N, w.nu, w.c, dtF, 3)
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------------------
 # the 128^3 / 16-slice FP64 workload (configuration 2)
 
 def cfg2(a):
@@ -603,9 +722,10 @@ def main():
                     help="cfg4: G variants solved beside the configured BF16 one (comma list of f64, f32, bf16; "
                          "':init64' for the initial coarse run in FP64, 'f64:spectral' for the FP64 RK4 coarse "
                          "map as a Fourier multiplier; '' for none)")
-    ap.add_argument("--workload", choices=["cfg4", "cfg2", "cfg5", "ref"], default="cfg4",
-                    help="cfg4: 384^3 x 64 slices, BF16 G (default); cfg2: 128^3 x 16 slices, FP64; "
-                         "cfg5: the 24-problem uneven batch")
+    ap.add_argument("--workload", choices=["cfg4", "cfg3", "cfg2", "cfg5", "ref"], default="cfg4",
+                    help="cfg4: 384^3 x 64 slices, BF16 G (default); cfg3: 256^3 x 32 slices, FP32 G; "
+                         "cfg2: 128^3 x 16 slices, FP64; cfg5: the 24-problem uneven batch; ref: the "
+                         "reference's own speedup job (128^3, Euler G, K = 3, 4..128 slices)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the batched RK4 step and a copy kernel of the same size (for profilers)")
     ap.add_argument("--reps", type=int, default=5, help="launches of the kernel with --roofline-only")
@@ -625,7 +745,7 @@ def main():
         sys.exit(2)
     if a.roofline_only:
         return roofline_only(a)
-    return {"cfg4": cfg4, "cfg2": cfg2, "cfg5": cfg5, "ref": refjob}[a.workload](a)
+    return {"cfg4": cfg4, "cfg3": cfg3, "cfg2": cfg2, "cfg5": cfg5, "ref": refjob}[a.workload](a)
 
 
 if __name__ == "__main__":
