@@ -683,7 +683,9 @@ def roofline_only(a):
     import stencilparareal_amd as sp
     from stencilparareal_amd.config import workload
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    w = workload(4 if a.workload == "cfg4" else 2)
+    if a.workload not in ("cfg4", "cfg3", "cfg2"):
+        sys.exit("--roofline-only runs the batched step of cfg4, cfg3 or cfg2")
+    w = workload({"cfg4": 4, "cfg3": 3, "cfg2": 2}[a.workload])
     N, S = w.N, a.slices or w.slices
     conf = conf_of(w)
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, 0.0, conf.cx, conf.cy, conf.cz)
