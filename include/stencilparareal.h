@@ -118,7 +118,8 @@ int sp_prop_set_upwind(sp_prop* p, int upwind);
 /* SP_PATH_FUSED (default) or SP_PATH_STAGED for DoRK4. */
 int sp_prop_set_path(sp_prop* p, int path);
 /* Fused-kernel choice: variant SP_KERNEL_*, tile tx x ty columns, nthreads per block,
- * kchunk planes per block (0: whole k extent).  variant SP_KERNEL_AUTO restores the
+ * kchunk planes per block (0: whole k extent; the quad variant runs an odd kchunk as
+ * kchunk + 1, with the same results).  variant SP_KERNEL_AUTO restores the
  * built-in choice.  Compiled shapes: scalar {32x32x1024, 32x24x1024, 24x24x1024,
  * 32x16x512, 16x16x256}, pair {32x32x512, 32x16x512, 16x16x256}, quad {32x32x512}
  * (the built-in choice for FP64 launches of 256 or more blocks; FP32 / BF16 storage takes
