@@ -679,12 +679,12 @@ def refjob(a):
 def roofline_only(a):
     """Only the dominant kernel and a copy kernel of the same bytes (the PMC scale), for
     profilers: a.reps batched steps of the workload's slices."""
+    if a.workload not in ("cfg4", "cfg3", "cfg2"):
+        sys.exit("--roofline-only runs the batched step of cfg4, cfg3 or cfg2")
     import torch
     import stencilparareal_amd as sp
     from stencilparareal_amd.config import workload
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    if a.workload not in ("cfg4", "cfg3", "cfg2"):
-        sys.exit("--roofline-only runs the batched step of cfg4, cfg3 or cfg2")
     w = workload({"cfg4": 4, "cfg3": 3, "cfg2": 2}[a.workload])
     N, S = w.N, a.slices or w.slices
     conf = conf_of(w)

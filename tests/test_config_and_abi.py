@@ -11,6 +11,7 @@ import json
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -118,3 +119,26 @@ def test_product_package_does_not_use_the_oracle():
                 text = open(os.path.join(dirpath, f)).read()
                 assert not re.search(r"^\s*(import|from)\s+oracle\b", text, flags=re.M), f
                 assert "sp_oracle" not in text and "libsp_oracle" not in text, f
+
+
+def _bench(*args):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args],
+                          capture_output=True, text=True, timeout=120, cwd=ROOT)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU per box: the slices are the batch index of a single device
+    r = _bench("--gpus", "2")
+    assert r.returncode == 2 and "one GPU only" in r.stderr
+
+
+def test_bench_roofline_only_refuses_workloads_it_does_not_run():
+    r = _bench("--roofline-only", "--workload", "cfg5")
+    assert r.returncode != 0 and "--roofline-only runs the batched step of cfg4, cfg3 or cfg2" in r.stderr
+
+
+def test_bench_lists_its_workloads():
+    r = _bench("--help")
+    assert r.returncode == 0
+    for w in ("cfg4", "cfg3", "cfg2", "cfg5", "ref"):
+        assert w in r.stdout
