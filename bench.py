@@ -585,6 +585,31 @@ def cfg5(a):
     torch.cuda.synchronize()
     t_tol = time.perf_counter() - t1
     worst = max(sp.rel_l2(qt[i], ref[i]) for i in range(len(ups)))
+    # the fine sweep of the timed runs' last solve: every problem's fine steps (grouped 2x2
+    # launches), timed apart from the chains by HIP events; 16 algorithmic bytes per update
+    fine = [b.iteration_times(k)[0] for k in range(rep.iterations)]
+    fine_s = sum(fine) / len(fine)
+    sweep_bytes = 16.0 * sum(u.n ** 3 * u.slices * b.info(i).steps_fine_per_slice for i, u in enumerate(ups))
+    state = sum(u.n ** 3 * u.slices for u in ups)
+    x = torch.empty(state, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        y.copy_(x)
+    e1.record()
+    e1.synchronize()
+    copy = 2.0 * 8 * state * 5 / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del x, y
+    achieved = sweep_bytes / fine_s / 1e9
+    rl = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+          "frac_of_copy": round(achieved / copy, 4), "copy_gbs": round(copy, 1),
+          "kernel": "the fine sweep of one iteration: grouped k_rk4_fused_quad launches over the 24 problems",
+          "avg_sweep_ms": round(fine_s * 1e3, 3), "bytes_per_sweep": sweep_bytes,
+          "chain_ms_per_iteration": round(1e3 * sum(b.iteration_times(k)[1] for k in range(rep.iterations))
+                                          / rep.iterations, 3)}
     out = {"metric": METRIC, "value": round(rep.lattice_updates / (ms / 1e3) / 1e9, 3), "unit": "GLUP/s",
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -600,7 +625,7 @@ def cfg5(a):
            "batch_to_tol": {"tol": 1e-6, "wall_s": round(t_tol, 4), "iterations": rep_t.iterations,
                             "max_rel_l2_err": float(f"{worst:.3e}"),
                             "speedup_vs_serial_fine": round(t_serial / t_tol, 3)},
-           "per_problem": per, "roofline": None, "cpu_baseline": None}
+           "per_problem": per, "roofline": rl, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
 

@@ -326,6 +326,9 @@ int sp_batch_destroy(sp_batch* b);
 int sp_batch_run(sp_batch* b, void* const* u0, void* const* q, sp_batch_report* rep);
 /* Problem p's relative L2 increments of the last run, one per iteration. */
 int sp_batch_increments(const sp_batch* b, int p, double* inc);
+/* Iteration k (0-based) of the last run, timed by HIP events: the fine sweep (every
+ * problem's fine steps) and the coarse chains with the corrections, in seconds. */
+int sp_batch_iteration_times(const sp_batch* b, int k, double* fine_s, double* chain_s);
 int sp_batch_problem_info(const sp_batch* b, int p, sp_batch_info* info);
 /* Problem p's serial FP64 fine solution: DoRK4(u0 -> q, 0, dtF_p, tsF_p * S_p). */
 int sp_batch_serial(sp_batch* b, int p, const void* u0, void* q);

@@ -40,7 +40,8 @@ EXPORTS = [
     "sp_parareal_destroy", "sp_parareal_begin", "sp_parareal_iterate", "sp_parareal_iteration_times",
     "sp_parareal_solution",
     "sp_parareal_run", "sp_parareal_serial",
-    "sp_batch_create", "sp_batch_destroy", "sp_batch_run", "sp_batch_increments", "sp_batch_problem_info",
+    "sp_batch_create", "sp_batch_destroy", "sp_batch_run", "sp_batch_increments", "sp_batch_iteration_times",
+    "sp_batch_problem_info",
     "sp_batch_serial",
 ]
 
@@ -147,6 +148,7 @@ _SIGS = {
     "sp_batch_destroy": (C.c_int, [_vp]),
     "sp_batch_run": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(BatchReport)]),
     "sp_batch_increments": (C.c_int, [_vp, C.c_int, _dp]),
+    "sp_batch_iteration_times": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "sp_batch_problem_info": (C.c_int, [_vp, C.c_int, C.POINTER(BatchInfo)]),
     "sp_batch_serial": (C.c_int, [_vp, C.c_int, _vp, _vp]),
 }

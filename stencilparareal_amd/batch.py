@@ -64,6 +64,13 @@ class Batch:
         L.call("sp_batch_increments", self.h, p, out)
         return list(out[:self.report.iterations])
 
+    def iteration_times(self, k: int) -> tuple[float, float]:
+        """(fine sweep, coarse chains with the corrections) of iteration k of the last run,
+        in seconds (HIP events)."""
+        f, c = C.c_double(), C.c_double()
+        L.call("sp_batch_iteration_times", self.h, k, C.byref(f), C.byref(c))
+        return f.value, c.value
+
     def info(self, p: int) -> L.BatchInfo:
         i = L.BatchInfo()
         L.call("sp_batch_problem_info", self.h, p, C.byref(i))

@@ -90,8 +90,12 @@ struct sp_batch {
     std::vector<int> corr_n;
     std::vector<std::vector<int>> init_plan;   // initial chain position j (1..maxS) -> groups
     std::vector<std::vector<double>> inc;  // per problem, per iteration
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr;   // the fine sweep of one iteration
     hipGraph_t graph = nullptr;
+    hipGraphExec_t gexecC = nullptr;  // the coarse chains and corrections of one iteration
+    hipGraph_t graphC = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<double> t_fine, t_chain;  // per iteration of the last run, milliseconds
     int iterations = 0;
 };
 
@@ -277,14 +281,39 @@ int build_plans(sp_batch* b)
     return SP_OK;
 }
 
-int iteration(sp_batch* b, hipStream_t s)
+// One iteration is the fine sweep (every problem's slices, one grouped launch per fine step
+// and tile shape), then the coarse chains with the corrections, position by position.
+int fine_sweep(sp_batch* b, hipStream_t s)
 {
     for (int g : b->fine_plan) SP_TRY(run_groups(b, s, g));
+    return SP_OK;
+}
+
+int chains(sp_batch* b, hipStream_t s)
+{
     for (int j = 0; j < b->maxS; ++j) {
         for (int g : b->chain_plan[j]) SP_TRY(run_groups(b, s, g));
         SP_TRY(launch_correction_delta_grouped(s, b->gdt, (const CorrRecord*)((const char*)b->arena.p + b->corr_rec[j]),
                                                b->corr_n[j], kCorrBlocks));
     }
+    return SP_OK;
+}
+
+// Records what fn launches on stream s into an instantiated graph.
+template <typename F>
+int capture(hipStream_t s, hipGraph_t* graph, hipGraphExec_t* exec, F fn)
+{
+    SP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rr = fn();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rr != SP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rr;
+    }
+    SP_HIP(e);
+    *graph = g;
+    SP_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     return SP_OK;
 }
 
@@ -297,6 +326,10 @@ void release(sp_batch* b)
     }
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->gexecC) (void)hipGraphExecDestroy(b->gexecC);
+    if (b->graphC) (void)hipGraphDestroy(b->graphC);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
     for (Prob& q : b->pr) {
         for (Buf* x : {&q.U, &q.F, &q.FS, &q.Gw0, &q.Gw1, &q.D, &q.Q}) x->release();
         if (q.prop) sp_prop_destroy(q.prop);
@@ -456,27 +489,33 @@ int sp_batch_run(sp_batch* b, void* const* u0, void* const* q, sp_batch_report* 
     // the stream the iterations run on: a graph needs a created stream, so with the null
     // stream as the context's stream it is replayed on the context's own one
     const hipStream_t ws = (b->cfg.use_graphs && !s) ? b->ctx->own : s;
+    for (hipEvent_t& e : b->ev)
+        if (!e) SP_HIP(hipEventCreate(&e));
+    b->t_fine.clear();
+    b->t_chain.clear();
     for (; k < b->cfg.kmax; ++k) {
+        // the fine sweep and the chains between HIP events (two graphs when captured)
+        SP_HIP(hipEventRecord(b->ev[0], ws));
         if (b->cfg.use_graphs) {
-            const hipStream_t cs = ws;
             if (!b->gexec) {
-                SP_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-                const int rr = iteration(b, cs);
-                hipGraph_t g = nullptr;
-                const hipError_t e = hipStreamEndCapture(cs, &g);
-                if (rr != SP_OK) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return rr;
-                }
-                SP_HIP(e);
-                b->graph = g;
-                SP_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+                SP_TRY(capture(ws, &b->graph, &b->gexec, [&] { return fine_sweep(b, ws); }));
+                SP_TRY(capture(ws, &b->graphC, &b->gexecC, [&] { return chains(b, ws); }));
             }
-            SP_HIP(hipGraphLaunch(b->gexec, cs));
-            SP_HIP(hipStreamSynchronize(cs));
+            SP_HIP(hipGraphLaunch(b->gexec, ws));
+            SP_HIP(hipEventRecord(b->ev[1], ws));
+            SP_HIP(hipGraphLaunch(b->gexecC, ws));
         } else {
-            SP_TRY(iteration(b, s));
+            SP_TRY(fine_sweep(b, ws));
+            SP_HIP(hipEventRecord(b->ev[1], ws));
+            SP_TRY(chains(b, ws));
         }
+        SP_HIP(hipEventRecord(b->ev[2], ws));
+        SP_HIP(hipEventSynchronize(b->ev[2]));
+        float f_ms = 0.f, c_ms = 0.f;
+        SP_HIP(hipEventElapsedTime(&f_ms, b->ev[0], b->ev[1]));
+        SP_HIP(hipEventElapsedTime(&c_ms, b->ev[1], b->ev[2]));
+        b->t_fine.push_back(f_ms);
+        b->t_chain.push_back(c_ms);
         SP_HIP(hipMemcpyAsync(b->part_host.data(), b->part.p, b->part_host.size() * sizeof(double),
                               hipMemcpyDeviceToHost, ws));
         SP_HIP(hipStreamSynchronize(ws));
@@ -514,6 +553,16 @@ int sp_batch_run(sp_batch* b, void* const* u0, void* const* q, sp_batch_report* 
         rep->t_total = now() - t0;
         rep->lattice_updates = lup;
     }
+    return SP_OK;
+}
+
+int sp_batch_iteration_times(const sp_batch* b, int k, double* fine_s, double* chain_s)
+{
+    SP_CHECK(b, SP_ERR_INVALID, "sp_batch_iteration_times: null handle");
+    SP_CHECK(k >= 0 && k < (int)b->t_fine.size(), SP_ERR_INVALID,
+             "sp_batch_iteration_times: iteration " + std::to_string(k) + " did not run in the last sp_batch_run");
+    if (fine_s) *fine_s = b->t_fine[k] * 1e-3;
+    if (chain_s) *chain_s = b->t_chain[k] * 1e-3;
     return SP_OK;
 }
 

@@ -57,6 +57,7 @@ def test_batch_fp64_coarse_matches_oracle_per_problem(sp, graphs):
         f.set(gauss(p.n, i))
     rep = b.run(u0, q)
     assert rep.iterations == K
+    assert all(f > 0 and c > 0 for f, c in (b.iteration_times(k) for k in range(K)))
     for i, p in enumerate(probs):
         conf = RuntimeConfiguration(gridSize=p.n, endTime=p.end_time, timeStepsFine=p.steps_fine,
                                     timeStepsCoarse=p.steps_coarse, nu0=p.nu, nufreq=0.0, cx=p.c[0], cy=p.c[1],
@@ -169,6 +170,13 @@ def test_cfg5_uneven_batch_as_stated(sp):
     u0, q = _uneven_fields(sp, b, ups)
     rep = b.run(u0, q)
     assert rep.iterations == K
+    # the fine sweep and the chains of every iteration, timed apart (two graphs)
+    times = [b.iteration_times(k) for k in range(K)]
+    print("fine sweep / chains per iteration (s)", ["%.4f / %.4f" % t for t in times])
+    assert all(f > 0 and c > 0 for f, c in times)
+    assert sum(f + c for f, c in times) <= rep.t_total
+    with pytest.raises(Exception):
+        b.iteration_times(K)
     inc = list(rep.increment[:K])
     print("largest increment per iteration", ["%.2e" % x for x in inc])
     # the largest increment over the problems falls every iteration until the FP64 floor
