@@ -556,6 +556,24 @@ def test_parareal_euler_spectral_coarse_vs_oracle(sp, graphs, upwind, nufreq):
         sp.Parareal(conv, init, sol, conf, coarse="euler_spectral", coarse_dtype="f32")
 
 
+@pytest.mark.parametrize("coarse", ["rk4_spectral", "euler_spectral"])
+def test_parareal_spectral_coarse_difference_form_vs_oracle(sp, coarse):
+    # the difference-form correction F(U^k) + G(U^{k+1} - U^k) with a spectral G: for a
+    # linear G the same update as the reference's G(U^{k+1}) + F(U^k) - G(U^k), so every
+    # iterate is the oracle's (standard form, stencil G) within 1e-12; with nu(t)
+    g = "rk4" if coarse == "rk4_spectral" else "euler"
+    conf, conv, u0, iters, serial = _parareal_case(sp, "fused", g, N=16, nufreq=80.0)
+    init = conv.field("u0")
+    init.set(u0)
+    sol = conv.field("sol")
+    pr = sp.Parareal(conv, init, sol, conf, coarse=coarse, correction="delta", use_graphs=True)
+    pr.begin()
+    assert relerr(pr.solution().get(), iters[0]) <= 1e-13
+    for k in range(1, conf.kmax + 1):
+        pr.iterate()
+        assert relerr(pr.solution().get(), iters[k]) <= 1e-12, k
+
+
 def test_parareal_autotuned_vs_oracle(sp):
     conf, conv, u0, iters, serial = _parareal_case(sp, "fused", "rk4")
     init, sol = conv.field(), conv.field()
