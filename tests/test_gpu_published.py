@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PUB = json.load(open(os.path.join(ROOT, "tests", "golden", "published.json")))
 
 
-def run_case(slices, nufreq, kmax):
+def run_case(slices, nufreq, kmax, coarse="euler"):
     import stencilparareal_amd as sp
     st = PUB["setup"]
     N = st["gridSize"]
@@ -38,7 +38,7 @@ def run_case(slices, nufreq, kmax):
     conv = sp.Convection(N, N, N, conf.dx(), conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, upwind=True)
     init, q, ref = conv.field(kind="stella"), conv.field(), conv.field()
     sp.fill_exact(init, conf.nu0, conf.nufreq, conf.cx, conf.cy, conf.cz, 0.0)
-    pr = sp.Parareal(conv, init, q, conf, coarse="euler")
+    pr = sp.Parareal(conv, init, q, conf, coarse=coarse)
     pr.q = ref
     pr.DoSerial()
     pr.q = q
@@ -67,11 +67,13 @@ def test_convergence_table(row):
     compare(errs, rec["error_k1_to_k8"])
 
 
-def test_speedup_run_errors_k3():
-    # 04-SpeedupGPU: K = 3 error for 4 and 16 slices (not in the convergence table)
+@pytest.mark.parametrize("coarse", ["euler", "euler_spectral"])
+def test_speedup_run_errors_k3(coarse):
+    # 04-SpeedupGPU: K = 3 error for 4 and 16 slices (not in the convergence table); the
+    # Euler coarse propagator as stencil steps and as its Fourier multiplier per segment
     for s in ("4", "16"):
         want = PUB["runs"]["04-SpeedupGPU"][s]["Error at end"]
-        errs = run_case(int(s), 0.0, 3)
+        errs = run_case(int(s), 0.0, 3, coarse)
         compare(errs[2:3], [want])
 
 
